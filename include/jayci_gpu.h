@@ -1,0 +1,250 @@
+/*
+ * jayci_gpu.h -- C ABI of libjayci_gpu: the MI355X sigma (H v) build and Davidson
+ * eigensolver of JayCI.
+ *
+ * Two layers:
+ *   1. Drop-in entry points with the reference's exact names and signatures.  A JayCI
+ *      build links them in place of action_util.o, cimapping.o, davidson.o,
+ *      citruncate.o and the output writers (reference Makefile link list of `jayci`).
+ *   2. A reentrant handle API (jci_*) underneath, for callers that keep several
+ *      solves resident on one GPU.
+ *
+ * All pointers are plain host pointers unless a parameter name ends in _dev (device
+ * memory).  Integrals use the reference's packed 1-based layout: moints1 has
+ * n(n+1)/2 entries addressed by index1e(i,j)-1 and moints2 m1(m1+1)/2 entries
+ * addressed by index2e(i,j,k,l)-1 (source/moindex.c:14-41).  Orbitals 1..n are bits
+ * 0..n-1 of a 64-bit string mask (n <= 64).  Determinant order is the reference's:
+ * (alpha group, beta group) blocks in citrunc order, alpha-major inside a block.
+ *
+ * Errors: entry points returning int give 0 on success and a nonzero code otherwise
+ * (-1 invalid argument or unsupported case, -2 HIP runtime failure, -3 out of memory,
+ * 1 Davidson not converged within maxiter); jci_last_error() describes the last failure
+ * of the calling thread, and the reference-named entry points also print it to stderr
+ * as "*** Error: <function>: <code>" like source/errorlib.c.
+ */
+#ifndef JAYCI_GPU_H
+#define JAYCI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#define JCI_RESTRICT
+#else
+#define JCI_RESTRICT restrict
+#endif
+
+/* ---- reference-format types -------------------------------------------------------- */
+
+/* source/include/binarystr.h:6-11 (byte1: internal-orbital mask; virtx: orbitals above
+ * ninto; byte2 unused) */
+struct occstr {
+    long long int byte1;
+    long long int byte2;
+    int virtx[2];
+    int nvrtx;
+};
+
+/* source/include/binarystr.h:14-19 */
+struct det {
+    struct occstr astr;
+    struct occstr bstr;
+    int cas;
+};
+
+/* source/include/cimapping.h:7-17 */
+struct section {
+    int first;
+    int last;
+    struct section *next;
+    struct section *prev;
+};
+
+struct rowmap {
+    int nsec;
+    struct section *sec;
+};
+
+/* ---- drop-in entry points (reference signatures) ------------------------------------- */
+
+/* replaces source/citruncate.c:123 (citrunc_rtnlist); serial truncation rule; returns a
+ * malloc'ed determinant list the caller frees with free() */
+struct det *citrunc_rtnlist(int aelec, int belec, int orbs, int nfrzc, int ndocc, int nactv,
+                            int nfrzv, int xlvl, int *astr_len, int *bstr_len, int *dtrm_len,
+                            int *ci_orbs, int *ci_aelec, int *ci_belec);
+
+/* replaces source/cimapping.c:92 (generate_cimap): fills hmap[0..ndets) (allocated by
+ * the caller) with the reference's section lists of connected j >= i, computed by the
+ * GPU screen; also keeps the connectivity on the device for compute_hv / dvdalg */
+int generate_cimap(struct det *dlist, int ndets, int nactv, struct rowmap *hmap);
+
+/* replaces source/cimapping.c:70 (deallocate_cimap): frees the section lists, hmap
+ * itself and the device copy */
+int deallocate_cimap(struct rowmap *hmap, int ndets);
+
+/* replaces source/action_util.c:106 (compute_hdgls) */
+void compute_hdgls(struct det *dlist, int ndets, double *moints1, double *moints2, int aelec,
+                   int belec, double *hdgls, int ninto);
+
+/* replaces source/action_util.c:204 (hmatels); corrected same-spin double rule of
+ * mpi_source/action_util.c:237-245 */
+double hmatels(struct det deti, struct det detj, double *moints1, double *moints2, int aelec,
+               int belec, int nactv);
+
+/* replaces source/action_util.c:121 (compute_hv): c += H v over the map in hmap */
+void compute_hv(struct det *dlist, int ndets, double *moints1, double *moints2, int aelec,
+                int belec, double *JCI_RESTRICT v, double *JCI_RESTRICT c, int ninto,
+                struct rowmap *hmap);
+
+/* replaces source/davidson.c:91 (dvdalg): lowest nroots eigenpairs; civec[r][0..ndets)
+ * (rows malloc'ed by the caller), cival[r] = eigenvalue + totfrze.  predr 1: guess from
+ * the leading refdim x refdim block; predr 2: read civector.dat in the working
+ * directory.  Returns 0, or a nonzero code (see above). */
+int dvdalg(struct det *dlist, int ndets, double *moints1, double *moints2, int aelec,
+           int belec, double *hdgls, int ninto, double totfrze, int maxiter, int krymin,
+           int krymax, int nroots, double restol, struct rowmap *hmap, double **civec,
+           double *cival, int predr, int plvl, int norbs, int refdim);
+
+/* replaces source/ioutil.c:184 (print_civectors): writes civector.dat */
+int print_civectors(int aelec, int belec, int orbs, double **civec, int ndets, int roots,
+                    double *cival);
+
+/* replaces source/ioutil.c:223 (read_civectors): reads civector.dat; 0 ok, 1 no file,
+ * 2 header mismatch */
+int read_civectors(int aelec, int belec, int orbs, double **civec, int ndets, int roots,
+                   double *cival);
+
+/* replaces source/write_wavefunction.c:12 (write_wavefunction): writes ciwvfcn.dat,
+ * reading &general from jayci.in */
+void write_wavefunction(struct det *detlist, int ndets, int nroots, double **civec,
+                        double *cival, int orbs, int ninto, int elec);
+
+/* replace source/ioutil.c:332 (readgeninput) and :280 (readdaiinput): namelists of
+ * jayci.in in the working directory (source/readnamelist.f90:38-41, defaults :79-85);
+ * restol is parsed at full precision */
+void readgeninput(int *elec, int *orbs, int *nfrozen, int *ndocc, int *nactive, int *xlevel,
+                  int *nfrzvirt, int *printlvl, int *printwvf, int *err);
+void readdaiinput(int *maxiter, int *krymin, int *krymax, int *nroots, int *prediagr,
+                  int *refdim, double *restol, int *err);
+
+/* ---- handle API ------------------------------------------------------------------------ */
+
+typedef struct jci_ctx jci_ctx;
+typedef struct jci_space jci_space;
+typedef struct jci_ham jci_ham;
+
+#define JCI_RULE_SERIAL 0   /* source/citruncate.c:291-312 */
+#define JCI_RULE_MRCISD 1   /* mpi_source/citruncate.c:414-471 */
+
+#define JCI_PAIRS_FP64 1    /* keep the connected-pair list with FP64 values */
+#define JCI_PAIRS_FP32 2    /* ... and/or with FP32 values */
+#define JCI_PAIRS_UPPER 4   /* store only j > i (symmetric sigma with atomics) */
+
+#define JCI_SIGMA_AUTO 0
+#define JCI_SIGMA_PAIRS64 1
+#define JCI_SIGMA_PAIRS32 2
+#define JCI_SIGMA_PAIRS_SYM64 3
+#define JCI_SIGMA_DIRECT64 4
+#define JCI_SIGMA_DIRECT32 5
+
+#define JCI_PREC_FP64 0
+#define JCI_PREC_MIXED 1
+
+#define JCI_GUESS_BLOCK 1   /* leading refdim block (source/initguess_sbd.c:24) */
+#define JCI_GUESS_USER 2    /* caller vectors (source/initguess_roldv.c:20) */
+#define JCI_GUESS_DIAG 3    /* block of the refdim lowest diagonal elements */
+
+#define JCI_MAXROOTS 8
+
+const char *jci_last_error(void);
+int jci_device_count(int *n);
+
+int jci_ctx_create(int device, jci_ctx **out);
+void jci_ctx_destroy(jci_ctx *ctx);
+int jci_ctx_sync(jci_ctx *ctx);
+/* the HIP stream the context's work is queued on (a hipStream_t) */
+void *jci_ctx_stream(jci_ctx *ctx);
+
+int jci_space_create(jci_ctx *ctx, int aelec, int belec, int norbs, int ndocc, int nactv,
+                     int xlvl, int rule, jci_space **out);
+void jci_space_destroy(jci_space *sp);
+long long jci_space_ndets(const jci_space *sp);
+long long jci_space_nstrings(const jci_space *sp, int spin);
+int jci_space_ngroups(const jci_space *sp, int spin);
+/* groups as rows (docc, actv, virt, start, count) */
+int jci_space_groups(const jci_space *sp, int spin, long long *out);
+int jci_space_nblocks(const jci_space *sp);
+/* blocks as rows (alpha group, beta group, first determinant) */
+int jci_space_blocks(const jci_space *sp, long long *out);
+int jci_space_strings(jci_space *sp, int spin, uint64_t *masks);
+/* alpha/beta masks of determinants first .. first+count-1 */
+int jci_space_dets(jci_space *sp, long long first, long long count, uint64_t *a, uint64_t *b);
+/* address of determinant (a, b), or -1 if it is not in the space */
+long long jci_space_index(jci_space *sp, uint64_t a, uint64_t b);
+
+int jci_ham_create(jci_space *sp, const double *moints1, const double *moints2, int flags,
+                   jci_ham **out);
+void jci_ham_destroy(jci_ham *h);
+long long jci_ham_npairs(const jci_ham *h);
+int jci_ham_hdiag(jci_ham *h, double *out);
+/* connected off-diagonal entries of the given rows (both triangles, unsorted):
+ * row t's entries at [t*cap, t*cap + counts[t]) */
+int jci_ham_rows(jci_ham *h, const long long *rows, int nrows, long long cap, long long *cols,
+                 double *vals, int *phase, int *deg, long long *counts);
+/* y = H x for nvec vectors stored determinant-major ([N][nvec]); _dev: device memory */
+int jci_sigma_dev(jci_ham *h, int method, int nvec, const double *x_dev, double *y_dev);
+int jci_sigma(jci_ham *h, int method, int nvec, const double *x, double *y);
+
+typedef struct {
+    int maxiter, krymin, krymax, nroots, refdim;
+    double restol;
+    int precision;        /* JCI_PREC_FP64 or JCI_PREC_MIXED */
+    double mixed_switch;  /* residual norm below which FP32 sigma hands over to FP64 */
+    int sigma_fp64, sigma_fp32; /* JCI_SIGMA_* used in each phase (0 = automatic) */
+    int guess;            /* JCI_GUESS_* */
+    double totfrze;       /* added to the eigenvalues (nuclear repulsion + frozen core) */
+    int verbose;
+} jci_solve_params;
+
+typedef struct {
+    int iterations, converged, nbasis;
+    long long nsigma64, nsigma32;   /* single-vector sigma applications per precision */
+    double rnorm[JCI_MAXROOTS];
+    double seconds, sigma_seconds;
+} jci_solve_stats;
+
+void jci_solve_defaults(jci_solve_params *p);
+/* lowest p->nroots eigenpairs; evecs (may be NULL) receives nroots vectors of N
+ * (vector-major); guess (JCI_GUESS_USER) holds krymin vectors of N */
+int jci_solve(jci_ham *h, const jci_solve_params *p, const double *guess, double *evals,
+              double *evecs, jci_solve_stats *stats);
+
+/* G = V^T W for tall-skinny blocks on the FP64 matrix cores (the Davidson projection
+ * kernel): V is m vectors of length n, W is w vectors (vector-major, host memory),
+ * m, w <= 64; G is m x w row-major */
+int jci_gram(jci_ctx *ctx, int m, int w, long long n, const double *V, const double *W,
+             double *G);
+
+/* Measurement helpers.  jci_bench_sigma times `reps` back-to-back sigma launches of the
+ * given method on nvec random vectors with HIP events recorded on the context stream
+ * (the stream the kernel runs on) and returns the mean milliseconds per launch.
+ * jci_bench_copy does the same for a plain device copy of `bytes` bytes (bandwidth
+ * probe and the byte-counter calibration of a profiler run). */
+int jci_bench_sigma(jci_ham *h, int method, int nvec, int reps, double *ms_per_launch);
+int jci_bench_copy(jci_ctx *ctx, long long bytes, int reps, double *ms_per_launch);
+
+/* dense symmetric eigensolver (cyclic Jacobi, FP64) used for the Davidson subspace:
+ * eigenvalues ascending, eigenvector i in vec[i*n + 0..n-1] (largest component positive) */
+int jci_jacobi(int n, const double *a, double *eval, double *vec);
+
+/* Synthetic integrals: h_ii = -2.0 + 0.25 i (1-based i), h_ij ~ N(0, 0.05^2);
+ * (ij|kl) = sum_{P=1..2n} L^P_ij L^P_kl with L^P symmetric, L^P_ii ~ U(0.1, 0.3),
+ * L^P_ij ~ N(0, 0.02^2).  SplitMix64 stream from `seed`, Box-Muller normals.
+ * E_nuc for these inputs is 10.0 (JCI_SYNTH_ENUC). */
+#define JCI_SYNTH_ENUC 10.0
+int jci_synth_integrals(int norbs, uint64_t seed, double *moints1, double *moints2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JAYCI_GPU_H */

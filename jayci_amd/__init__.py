@@ -1,0 +1,359 @@
+"""jayci_amd: MI355X sigma (H v) build and Davidson eigensolver of JayCI.
+
+Python mirror of the C ABI in include/jayci_gpu.h (libjayci_gpu.so, built in-tree by
+jayci_amd.build).  The reference path this replaces is source/davidson.c:91 (dvdalg) and
+source/action_util.c:121 (compute_hv) with their determinant-space and integral inputs.
+Everything here runs on the GPU through the library; there is no CPU fallback: if the
+library is missing, importing raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjayci_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "jayci_gpu.h")
+
+RULE_SERIAL, RULE_MRCISD = 0, 1
+PAIRS_FP64, PAIRS_FP32, PAIRS_UPPER = 1, 2, 4
+SIGMA_AUTO, SIGMA_PAIRS64, SIGMA_PAIRS32, SIGMA_PAIRS_SYM64, SIGMA_DIRECT64, SIGMA_DIRECT32 = range(6)
+PREC_FP64, PREC_MIXED = 0, 1
+GUESS_BLOCK, GUESS_USER, GUESS_DIAG = 1, 2, 3
+MAXROOTS = 8
+SYNTH_ENUC = 10.0
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("jayci_amd: %s is not built (run python jayci_amd/build.py)" % LIB_PATH)
+
+_L = ctypes.CDLL(LIB_PATH)
+_P = ctypes.c_void_p
+_i, _ll, _d = ctypes.c_int, ctypes.c_longlong, ctypes.c_double
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [("maxiter", _i), ("krymin", _i), ("krymax", _i), ("nroots", _i), ("refdim", _i),
+                ("restol", _d), ("precision", _i), ("mixed_switch", _d), ("sigma_fp64", _i),
+                ("sigma_fp32", _i), ("guess", _i), ("totfrze", _d), ("verbose", _i)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", _i), ("converged", _i), ("nbasis", _i), ("nsigma64", _ll),
+                ("nsigma32", _ll), ("rnorm", _d * MAXROOTS), ("seconds", _d),
+                ("sigma_seconds", _d)]
+
+
+def _sig(name, res, args):
+    f = getattr(_L, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("jci_last_error", ctypes.c_char_p, [])
+_sig("jci_device_count", _i, [ctypes.POINTER(_i)])
+_sig("jci_ctx_create", _i, [_i, ctypes.POINTER(_P)])
+_sig("jci_ctx_destroy", None, [_P])
+_sig("jci_ctx_sync", _i, [_P])
+_sig("jci_ctx_stream", _P, [_P])
+_sig("jci_space_create", _i, [_P, _i, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_P)])
+_sig("jci_space_destroy", None, [_P])
+_sig("jci_space_ndets", _ll, [_P])
+_sig("jci_space_nstrings", _ll, [_P, _i])
+_sig("jci_space_ngroups", _i, [_P, _i])
+_sig("jci_space_groups", _i, [_P, _i, _P])
+_sig("jci_space_nblocks", _i, [_P])
+_sig("jci_space_blocks", _i, [_P, _P])
+_sig("jci_space_strings", _i, [_P, _i, _P])
+_sig("jci_space_dets", _i, [_P, _ll, _ll, _P, _P])
+_sig("jci_space_index", _ll, [_P, ctypes.c_uint64, ctypes.c_uint64])
+_sig("jci_ham_create", _i, [_P, _P, _P, _i, ctypes.POINTER(_P)])
+_sig("jci_ham_destroy", None, [_P])
+_sig("jci_ham_npairs", _ll, [_P])
+_sig("jci_ham_hdiag", _i, [_P, _P])
+_sig("jci_ham_rows", _i, [_P, _P, _i, _ll, _P, _P, _P, _P, _P])
+_sig("jci_sigma_dev", _i, [_P, _i, _i, _P, _P])
+_sig("jci_sigma", _i, [_P, _i, _i, _P, _P])
+_sig("jci_solve_defaults", None, [ctypes.POINTER(SolveParams)])
+_sig("jci_solve", _i, [_P, ctypes.POINTER(SolveParams), _P, _P, _P, ctypes.POINTER(SolveStats)])
+_sig("jci_synth_integrals", _i, [_i, ctypes.c_uint64, _P, _P])
+_sig("jci_jacobi", _i, [_i, _P, _P, _P])
+_sig("jci_gram", _i, [_P, _i, _i, _ll, _P, _P, _P])
+_sig("jci_bench_sigma", _i, [_P, _i, _i, _i, ctypes.POINTER(_d)])
+_sig("jci_bench_copy", _i, [_P, _ll, _i, ctypes.POINTER(_d)])
+
+
+class JciError(RuntimeError):
+    pass
+
+
+def last_error():
+    return _L.jci_last_error().decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise JciError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def lib():
+    return _L
+
+
+def header_functions():
+    """Names of the functions include/jayci_gpu.h declares."""
+    import re
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
+    skip = {"if", "sizeof", "defined", "struct", "return"}
+    decl = []
+    for m in re.finditer(r"^[A-Za-z_][^;#{}]*?\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", txt,
+                         flags=re.M | re.S):
+        decl.append(m.group(1))
+    return sorted(set(d for d in decl if d not in skip))
+
+
+def m1_len(n):
+    return n * (n + 1) // 2
+
+
+def m2_len(n):
+    a = m1_len(n)
+    return a * (a + 1) // 2
+
+
+def synth_integrals(norbs, seed):
+    """Synthetic packed integrals (see jci_synth_integrals in include/jayci_gpu.h)."""
+    m1 = np.zeros(m1_len(norbs))
+    m2 = np.zeros(m2_len(norbs))
+    _check(_L.jci_synth_integrals(norbs, seed, _p(m1), _p(m2)), "jci_synth_integrals")
+    return m1, m2
+
+
+def jacobi(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    ev = np.zeros(n)
+    vec = np.zeros((n, n))
+    _check(_L.jci_jacobi(n, _p(a), _p(ev), _p(vec)), "jci_jacobi")
+    return ev, vec
+
+
+def device_count():
+    n = _i()
+    _L.jci_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _P()
+        _check(_L.jci_ctx_create(device, ctypes.byref(h)), "jci_ctx_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _L.jci_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        _check(_L.jci_ctx_sync(self._h), "jci_ctx_sync")
+
+    @property
+    def stream(self):
+        return _L.jci_ctx_stream(self._h)
+
+    def bench_copy(self, nbytes, reps=20):
+        """Mean ms of a device copy of nbytes (HIP events on the context stream)."""
+        ms = _d()
+        _check(_L.jci_bench_copy(self._h, nbytes, reps, ctypes.byref(ms)), "jci_bench_copy")
+        return ms.value
+
+    def gram(self, V, W):
+        """V^T W on the FP64 matrix cores; V (m, n), W (w, n)."""
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        G = np.zeros((V.shape[0], W.shape[0]))
+        _check(_L.jci_gram(self._h, V.shape[0], W.shape[0], V.shape[1], _p(V), _p(W), _p(G)),
+               "jci_gram")
+        return G
+
+
+class Space:
+    """Determinant space: alpha/beta string tables on the device (reference order)."""
+
+    def __init__(self, ctx, na, nb, norbs, ndocc=0, nactv=None, xlvl=0, rule=RULE_MRCISD):
+        if nactv is None:
+            nactv = norbs - ndocc
+        self.ctx = ctx
+        self.na, self.nb, self.norbs = na, nb, norbs
+        self.ndocc, self.nactv, self.xlvl, self.rule = ndocc, nactv, xlvl, rule
+        h = _P()
+        _check(_L.jci_space_create(ctx._h, na, nb, norbs, ndocc, nactv, xlvl, rule, ctypes.byref(h)),
+               "jci_space_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _L.jci_space_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def ndets(self):
+        return _L.jci_space_ndets(self._h)
+
+    def nstrings(self, spin):
+        return _L.jci_space_nstrings(self._h, spin)
+
+    def groups(self, spin):
+        n = _L.jci_space_ngroups(self._h, spin)
+        out = np.zeros((n, 5), dtype=np.int64)
+        _L.jci_space_groups(self._h, spin, _p(out))
+        return out
+
+    def blocks(self):
+        n = _L.jci_space_nblocks(self._h)
+        out = np.zeros((n, 3), dtype=np.int64)
+        _L.jci_space_blocks(self._h, _p(out))
+        return out
+
+    def strings(self, spin):
+        out = np.zeros(self.nstrings(spin), dtype=np.uint64)
+        _check(_L.jci_space_strings(self._h, spin, _p(out)), "jci_space_strings")
+        return out
+
+    def dets(self, first=0, count=None):
+        if count is None:
+            count = self.ndets - first
+        a = np.zeros(count, dtype=np.uint64)
+        b = np.zeros(count, dtype=np.uint64)
+        _check(_L.jci_space_dets(self._h, first, count, _p(a), _p(b)), "jci_space_dets")
+        return a, b
+
+    def index(self, a, b):
+        return _L.jci_space_index(self._h, int(a), int(b))
+
+
+class Hamiltonian:
+    """Integrals + diagonal on the device, with an optional materialised pair list."""
+
+    def __init__(self, space, m1, m2, flags=0):
+        self.space = space
+        self.m1 = np.ascontiguousarray(m1, dtype=np.float64)
+        self.m2 = np.ascontiguousarray(m2, dtype=np.float64)
+        if len(self.m1) < m1_len(space.norbs) or len(self.m2) < m2_len(space.norbs):
+            raise ValueError("integral arrays too short for %d orbitals" % space.norbs)
+        h = _P()
+        _check(_L.jci_ham_create(space._h, _p(self.m1), _p(self.m2), flags, ctypes.byref(h)),
+               "jci_ham_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _L.jci_ham_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def npairs(self):
+        return _L.jci_ham_npairs(self._h)
+
+    def hdiag(self):
+        out = np.zeros(self.space.ndets)
+        _check(_L.jci_ham_hdiag(self._h, _p(out)), "jci_ham_hdiag")
+        return out
+
+    def rows(self, rows, cap=None):
+        """Connected entries of each row: list of (cols, vals, phase, deg) sorted by column."""
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        if cap is None:
+            na, nb, n = self.space.na, self.space.nb, self.space.norbs
+            from math import comb
+            sa = na * (n - na)
+            sb = nb * (n - nb)
+            cap = sa + sb + comb(na, 2) * comb(n - na, 2) + comb(nb, 2) * comb(n - nb, 2) + sa * sb + 1
+        nr = len(rows)
+        cols = np.zeros(nr * cap, dtype=np.int64)
+        vals = np.zeros(nr * cap)
+        ph = np.zeros(nr * cap, dtype=np.int32)
+        dg = np.zeros(nr * cap, dtype=np.int32)
+        cnt = np.zeros(nr, dtype=np.int64)
+        _check(_L.jci_ham_rows(self._h, _p(rows), nr, cap, _p(cols), _p(vals), _p(ph), _p(dg),
+                               _p(cnt)), "jci_ham_rows")
+        out = []
+        for t in range(nr):
+            c = int(cnt[t])
+            if c > cap:
+                raise JciError("row capacity exceeded")
+            sl = slice(t * cap, t * cap + c)
+            o = np.argsort(cols[sl], kind="stable")
+            out.append((cols[sl][o], vals[sl][o], ph[sl][o], dg[sl][o]))
+        return out
+
+    def bench_sigma(self, method=SIGMA_AUTO, nvec=1, reps=20):
+        """Mean ms per sigma launch (HIP events on the context stream)."""
+        ms = _d()
+        _check(_L.jci_bench_sigma(self._h, method, nvec, reps, ctypes.byref(ms)), "jci_bench_sigma")
+        return ms.value
+
+    def sigma(self, x, method=SIGMA_AUTO):
+        """H x for x of shape (N,) or (k, N) (vector-major); returns the same shape."""
+        x = np.asarray(x, dtype=np.float64)
+        single = x.ndim == 1
+        X = x.reshape(1, -1) if single else x
+        k, N = X.shape
+        if N != self.space.ndets:
+            raise ValueError("vector length %d != ndets %d" % (N, self.space.ndets))
+        Xi = np.ascontiguousarray(X.T)          # determinant-major [N][k]
+        Yi = np.zeros_like(Xi)
+        _check(_L.jci_sigma(self._h, method, k, _p(Xi), _p(Yi)), "jci_sigma")
+        Y = np.ascontiguousarray(Yi.T)
+        return Y[0] if single else Y
+
+    def solve(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
+              precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, guess_vectors=None,
+              totfrze=0.0, sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, vectors=True,
+              verbose=False, check=True):
+        p = SolveParams()
+        _L.jci_solve_defaults(ctypes.byref(p))
+        p.nroots = nroots
+        p.restol = restol
+        p.maxiter = maxiter
+        p.krymin = krymin if krymin is not None else max(nroots + 1, min(4, self.space.ndets))
+        p.krymax = krymax if krymax is not None else min(max(p.krymin + 2 * nroots, 16), 64)
+        p.refdim = refdim
+        p.precision = precision
+        p.mixed_switch = mixed_switch
+        p.guess = guess
+        p.totfrze = totfrze
+        p.sigma_fp64 = sigma_fp64
+        p.sigma_fp32 = sigma_fp32
+        p.verbose = 1 if verbose else 0
+        N = self.space.ndets
+        ev = np.zeros(nroots)
+        vec = np.zeros((nroots, N)) if vectors else None
+        g = None
+        if guess == GUESS_USER:
+            g = np.ascontiguousarray(guess_vectors, dtype=np.float64)
+        st = SolveStats()
+        rc = _L.jci_solve(self._h, ctypes.byref(p), _p(g), _p(ev), _p(vec), ctypes.byref(st))
+        if check and rc != 0:
+            raise JciError("jci_solve failed (%d): %s" % (rc, last_error()))
+        stats = {"rc": rc, "iterations": st.iterations, "converged": bool(st.converged),
+                 "nbasis": st.nbasis, "nsigma64": st.nsigma64, "nsigma32": st.nsigma32,
+                 "rnorm": [st.rnorm[i] for i in range(nroots)], "seconds": st.seconds,
+                 "sigma_seconds": st.sigma_seconds}
+        return ev, vec, stats

@@ -1,0 +1,1000 @@
+// Handle API of libjayci_gpu: device context, determinant space, Hamiltonian (integrals,
+// diagonal, optional pair list), sigma dispatch and the block Davidson driver.
+#include "jci_internal.h"
+#include "slater.h"
+#include "../../include/jayci_gpu.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <algorithm>
+#include <vector>
+
+// ---- errors ---------------------------------------------------------------------------
+static thread_local char g_err[1024];
+
+void jci_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+JCI_API const char *jci_last_error(void) { return g_err; }
+
+#define RCHECK(x)                      \
+    do {                               \
+        int rc_ = (x);                 \
+        if (rc_) return rc_;           \
+    } while (0)
+
+static double now_s()
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+// ---- extra kernels used here (defined in kernels.hip) ----------------------------------
+int jk_det_masks(const SpaceDesc *d_desc, const uint64_t *ma, const uint64_t *mb,
+                 const long long *idx, long long first, long long count, uint64_t *oa,
+                 uint64_t *ob, hipStream_t st);
+int jk_rotate_inplace(int m, int kc, long long N, double *V, long long ld, const double *Y,
+                      hipStream_t st);
+int jk_scatter_guess(int nb, int nvec, const long long *idx, const double *Yb, double *V,
+                     long long ld, hipStream_t st);
+int jk_index_of(const SpaceDesc *d_desc, const unsigned long long *binom, uint64_t a,
+                uint64_t b, long long *out, hipStream_t st);
+int jk_copy(long long bytes, const void *src, void *dst, hipStream_t st);
+int jk_fill_random(long long n, unsigned long long seed, double *x, hipStream_t st);
+
+// ---- context ----------------------------------------------------------------------------
+
+
+JCI_API int jci_device_count(int *n)
+{
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *n = c;
+    return 0;
+}
+
+JCI_API int jci_ctx_create(int device, jci_ctx **out)
+{
+    *out = nullptr;
+    int n = 0;
+    jci_device_count(&n);
+    if (n <= 0) { jci_set_error("no HIP device visible"); return -2; }
+    if (device < 0 || device >= n) { jci_set_error("device %d out of range", device); return -1; }
+    JCI_CHECK(hipSetDevice(device));
+    jci_ctx *c = (jci_ctx *)calloc(1, sizeof(jci_ctx));
+    c->device = device;
+    JCI_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    JCI_CHECK(hipMalloc(&c->d_partial, sizeof(double) * JCI_PARTIAL_BLOCKS * JCI_MAXBASIS * JCI_MAXBASIS));
+    JCI_CHECK(hipMalloc(&c->d_small, sizeof(double) * JCI_MAXBASIS * JCI_MAXBASIS));
+    JCI_CHECK(hipHostMalloc(&c->h_small, sizeof(double) * JCI_MAXBASIS * JCI_MAXBASIS, 0));
+    std::vector<unsigned long long> tab(65 * 65);
+    jci_binom_table(tab.data());
+    JCI_CHECK(hipMalloc(&c->d_binom, sizeof(unsigned long long) * 65 * 65));
+    JCI_CHECK(hipMemcpy(c->d_binom, tab.data(), sizeof(unsigned long long) * 65 * 65,
+                        hipMemcpyHostToDevice));
+    *out = c;
+    return 0;
+}
+
+static void ctx_free(jci_ctx *c)
+{
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipFree(c->d_partial);
+    hipFree(c->d_small);
+    hipFree(c->d_binom);
+    hipHostFree(c->h_small);
+    hipStreamDestroy(c->stream);
+    free(c);
+}
+
+// a context is freed when it was destroyed and no space refers to it any more
+JCI_API void jci_ctx_destroy(jci_ctx *c)
+{
+    if (!c) return;
+    if (c->refs > 0) { c->closing = 1; return; }
+    ctx_free(c);
+}
+
+JCI_API int jci_ctx_sync(jci_ctx *c)
+{
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+JCI_API void *jci_ctx_stream(jci_ctx *c) { return (void *)c->stream; }
+
+// ---- space --------------------------------------------------------------------------------
+struct jci_space {
+    jci_ctx *ctx;
+    int refs, closing;          // Hamiltonians holding this space
+    SpaceDesc h;
+    SpaceDesc *d_desc;
+    uint64_t *d_mask[2];
+    int na, nb, xlvl, rule;
+};
+
+JCI_API int jci_space_create(jci_ctx *ctx, int aelec, int belec, int norbs, int ndocc,
+                                int nactv, int xlvl, int rule, jci_space **out)
+{
+    *out = nullptr;
+    jci_space *s = (jci_space *)calloc(1, sizeof(jci_space));
+    s->ctx = ctx; s->na = aelec; s->nb = belec; s->xlvl = xlvl; s->rule = rule;
+    if (jci_build_desc(aelec, belec, norbs, ndocc, nactv, xlvl, rule, &s->h)) { free(s); return -1; }
+    if (s->h.ndets <= 0) { free(s); jci_set_error("space: no determinants"); return -1; }
+    JCI_CHECK(hipSetDevice(ctx->device));
+    JCI_CHECK(hipMalloc(&s->d_desc, sizeof(SpaceDesc)));
+    JCI_CHECK(hipMemcpyAsync(s->d_desc, &s->h, sizeof(SpaceDesc), hipMemcpyHostToDevice, ctx->stream));
+    for (int sp = 0; sp < 2; sp++) {
+        long long n = s->h.s[sp].gstart[s->h.s[sp].ngrp];
+        JCI_CHECK(hipMalloc(&s->d_mask[sp], sizeof(uint64_t) * (n > 0 ? n : 1)));
+        RCHECK(jk_gen_strings(s->d_desc, s->h, sp, ctx->d_binom, s->d_mask[sp], ctx->stream));
+    }
+    JCI_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->refs++;
+    *out = s;
+    return 0;
+}
+
+static void space_free(jci_space *s)
+{
+    hipSetDevice(s->ctx->device);
+    hipStreamSynchronize(s->ctx->stream);
+    hipFree(s->d_desc);
+    hipFree(s->d_mask[0]);
+    hipFree(s->d_mask[1]);
+    jci_ctx *c = s->ctx;
+    free(s);
+    if (--c->refs == 0 && c->closing) ctx_free(c);
+}
+
+// a space is freed when it was destroyed and no Hamiltonian refers to it any more
+JCI_API void jci_space_destroy(jci_space *s)
+{
+    if (!s) return;
+    if (s->refs > 0) { s->closing = 1; return; }
+    space_free(s);
+}
+
+JCI_API long long jci_space_ndets(const jci_space *s) { return s->h.ndets; }
+JCI_API long long jci_space_nstrings(const jci_space *s, int spin)
+{
+    return s->h.s[spin].gstart[s->h.s[spin].ngrp];
+}
+JCI_API int jci_space_ngroups(const jci_space *s, int spin) { return s->h.s[spin].ngrp; }
+JCI_API int jci_space_groups(const jci_space *s, int spin, long long *out)
+{
+    const SpinDesc &S = s->h.s[spin];
+    for (int g = 0; g < S.ngrp; g++) {
+        out[5 * g] = S.gd[g]; out[5 * g + 1] = S.ga[g]; out[5 * g + 2] = S.gv[g];
+        out[5 * g + 3] = S.gstart[g]; out[5 * g + 4] = S.gn[g];
+    }
+    return 0;
+}
+JCI_API int jci_space_nblocks(const jci_space *s) { return s->h.nblk; }
+JCI_API int jci_space_blocks(const jci_space *s, long long *out)
+{
+    for (int b = 0; b < s->h.nblk; b++) {
+        out[3 * b] = s->h.bp[b]; out[3 * b + 1] = s->h.bq[b]; out[3 * b + 2] = s->h.boff[b];
+    }
+    return 0;
+}
+
+JCI_API int jci_space_strings(jci_space *s, int spin, uint64_t *masks)
+{
+    JCI_CHECK(hipSetDevice(s->ctx->device));
+    long long n = jci_space_nstrings(s, spin);
+    JCI_CHECK(hipMemcpyAsync(masks, s->d_mask[spin], sizeof(uint64_t) * n, hipMemcpyDeviceToHost,
+                             s->ctx->stream));
+    JCI_CHECK(hipStreamSynchronize(s->ctx->stream));
+    return 0;
+}
+
+JCI_API int jci_space_dets(jci_space *s, long long first, long long count, uint64_t *a,
+                              uint64_t *b)
+{
+    if (first < 0 || count < 0 || first + count > s->h.ndets) {
+        jci_set_error("space_dets: range out of bounds");
+        return -1;
+    }
+    if (count == 0) return 0;
+    JCI_CHECK(hipSetDevice(s->ctx->device));
+    uint64_t *d;
+    JCI_CHECK(hipMalloc(&d, sizeof(uint64_t) * 2 * count));
+    RCHECK(jk_det_masks(s->d_desc, s->d_mask[0], s->d_mask[1], nullptr, first, count, d,
+                        d + count, s->ctx->stream));
+    JCI_CHECK(hipMemcpyAsync(a, d, sizeof(uint64_t) * count, hipMemcpyDeviceToHost, s->ctx->stream));
+    JCI_CHECK(hipMemcpyAsync(b, d + count, sizeof(uint64_t) * count, hipMemcpyDeviceToHost,
+                             s->ctx->stream));
+    JCI_CHECK(hipStreamSynchronize(s->ctx->stream));
+    JCI_CHECK(hipFree(d));
+    return 0;
+}
+
+JCI_API long long jci_space_index(jci_space *s, uint64_t a, uint64_t b)
+{
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return -1;
+    long long *d, r = -1;
+    if (hipMalloc(&d, sizeof(long long)) != hipSuccess) return -1;
+    if (jk_index_of(s->d_desc, s->ctx->d_binom, a, b, d, s->ctx->stream) == 0) {
+        hipMemcpyAsync(&r, d, sizeof(long long), hipMemcpyDeviceToHost, s->ctx->stream);
+        hipStreamSynchronize(s->ctx->stream);
+    }
+    hipFree(d);
+    return r;
+}
+
+// ---- Hamiltonian --------------------------------------------------------------------------
+struct jci_ham {
+    jci_space *sp;
+    double *d_m1, *d_m2;
+    double *d_hd;
+    long long *d_rowptr;
+    int *d_col;
+    double *d_val64;
+    float *d_val32;
+    long long nnz;
+    int upper;
+    float *d_xf;             // FP32 staging for sigma inputs
+    long long xf_len;
+};
+
+static size_t m1_len(int n) { return (size_t)n * (n + 1) / 2; }
+static size_t m2_len(int n) { size_t a = m1_len(n); return a * (a + 1) / 2; }
+
+static int build_pairs(jci_ham *h, int flags)
+{
+    jci_space *s = h->sp;
+    jci_ctx *c = s->ctx;
+    long long N = s->h.ndets;
+    if (N >= (1LL << 31) - 1) {
+        jci_set_error("pair list: %lld determinants exceed 32-bit column indices", N);
+        return -1;
+    }
+    h->upper = (flags & JCI_PAIRS_UPPER) ? 1 : 0;
+    long long *d_cnt;
+    JCI_CHECK(hipMalloc(&d_cnt, sizeof(long long) * N));
+    JCI_CHECK(hipMalloc(&h->d_rowptr, sizeof(long long) * (N + 1)));
+    RCHECK(jk_row_count(s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom, d_cnt, h->upper,
+                        c->stream));
+    RCHECK(jk_scan(d_cnt, h->d_rowptr, N, c->stream));
+    JCI_CHECK(hipMemcpyAsync(&h->nnz, h->d_rowptr + N, sizeof(long long), hipMemcpyDeviceToHost,
+                             c->stream));
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    JCI_CHECK(hipFree(d_cnt));
+    size_t nz = (size_t)(h->nnz > 0 ? h->nnz : 1);
+    if (hipMalloc(&h->d_col, sizeof(int) * nz) != hipSuccess) {
+        jci_set_error("pair list: out of device memory (%lld pairs)", h->nnz);
+        return -3;
+    }
+    if ((flags & JCI_PAIRS_FP64) && hipMalloc(&h->d_val64, sizeof(double) * nz) != hipSuccess) {
+        jci_set_error("pair list: out of device memory (%lld FP64 values)", h->nnz);
+        return -3;
+    }
+    if ((flags & JCI_PAIRS_FP32) && hipMalloc(&h->d_val32, sizeof(float) * nz) != hipSuccess) {
+        jci_set_error("pair list: out of device memory (%lld FP32 values)", h->nnz);
+        return -3;
+    }
+    RCHECK(jk_row_fill(s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom, h->d_m1, h->d_m2,
+                       h->d_rowptr, h->d_col, h->d_val64, h->d_val32, h->upper, c->stream));
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+JCI_API int jci_ham_create(jci_space *s, const double *m1, const double *m2, int flags,
+                              jci_ham **out)
+{
+    *out = nullptr;
+    jci_ctx *c = s->ctx;
+    JCI_CHECK(hipSetDevice(c->device));
+    jci_ham *h = (jci_ham *)calloc(1, sizeof(jci_ham));
+    h->sp = s;
+    s->refs++;
+    int n = s->h.norbs;
+    JCI_CHECK(hipMalloc(&h->d_m1, sizeof(double) * m1_len(n)));
+    JCI_CHECK(hipMalloc(&h->d_m2, sizeof(double) * m2_len(n)));
+    JCI_CHECK(hipMemcpyAsync(h->d_m1, m1, sizeof(double) * m1_len(n), hipMemcpyHostToDevice, c->stream));
+    JCI_CHECK(hipMemcpyAsync(h->d_m2, m2, sizeof(double) * m2_len(n), hipMemcpyHostToDevice, c->stream));
+    JCI_CHECK(hipMalloc(&h->d_hd, sizeof(double) * s->h.ndets));
+    RCHECK(jk_hdiag(s->d_desc, s->h.ndets, s->d_mask[0], s->d_mask[1], h->d_m1, h->d_m2, h->d_hd,
+                    c->stream));
+    if (flags & (JCI_PAIRS_FP64 | JCI_PAIRS_FP32)) {
+        int rc = build_pairs(h, flags);
+        if (rc) { jci_ham_destroy(h); return rc; }
+    }
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    *out = h;
+    return 0;
+}
+
+JCI_API void jci_ham_destroy(jci_ham *h)
+{
+    if (!h) return;
+    hipSetDevice(h->sp->ctx->device);
+    hipStreamSynchronize(h->sp->ctx->stream);
+    hipFree(h->d_m1); hipFree(h->d_m2); hipFree(h->d_hd);
+    hipFree(h->d_rowptr); hipFree(h->d_col); hipFree(h->d_val64); hipFree(h->d_val32);
+    hipFree(h->d_xf);
+    jci_space *s = h->sp;
+    free(h);
+    if (--s->refs == 0 && s->closing) space_free(s);
+}
+
+JCI_API long long jci_ham_npairs(const jci_ham *h) { return h->nnz; }
+
+JCI_API int jci_ham_hdiag(jci_ham *h, double *out)
+{
+    jci_ctx *c = h->sp->ctx;
+    JCI_CHECK(hipSetDevice(c->device));
+    JCI_CHECK(hipMemcpyAsync(out, h->d_hd, sizeof(double) * h->sp->h.ndets, hipMemcpyDeviceToHost,
+                             c->stream));
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+JCI_API int jci_ham_rows(jci_ham *h, const long long *rows, int nrows, long long cap,
+                            long long *cols, double *vals, int *phase, int *deg, long long *counts)
+{
+    jci_space *s = h->sp;
+    jci_ctx *c = s->ctx;
+    if (nrows <= 0) return 0;
+    for (int t = 0; t < nrows; t++)
+        if (rows[t] < 0 || rows[t] >= s->h.ndets) { jci_set_error("ham_rows: row out of range"); return -1; }
+    JCI_CHECK(hipSetDevice(c->device));
+    size_t tot = (size_t)nrows * cap;
+    long long *d_rows, *d_cols, *d_cnt;
+    double *d_vals;
+    int *d_ph, *d_dg;
+    JCI_CHECK(hipMalloc(&d_rows, sizeof(long long) * nrows));
+    JCI_CHECK(hipMalloc(&d_cols, sizeof(long long) * tot));
+    JCI_CHECK(hipMalloc(&d_vals, sizeof(double) * tot));
+    JCI_CHECK(hipMalloc(&d_ph, sizeof(int) * tot));
+    JCI_CHECK(hipMalloc(&d_dg, sizeof(int) * tot));
+    JCI_CHECK(hipMalloc(&d_cnt, sizeof(long long) * nrows));
+    JCI_CHECK(hipMemcpyAsync(d_rows, rows, sizeof(long long) * nrows, hipMemcpyHostToDevice, c->stream));
+    RCHECK(jk_rows_debug(s->d_desc, s->d_mask[0], s->d_mask[1], c->d_binom, h->d_m1, h->d_m2,
+                         d_rows, nrows, cap, d_cols, d_vals, d_ph, d_dg, d_cnt, c->stream));
+    JCI_CHECK(hipMemcpyAsync(cols, d_cols, sizeof(long long) * tot, hipMemcpyDeviceToHost, c->stream));
+    JCI_CHECK(hipMemcpyAsync(vals, d_vals, sizeof(double) * tot, hipMemcpyDeviceToHost, c->stream));
+    JCI_CHECK(hipMemcpyAsync(phase, d_ph, sizeof(int) * tot, hipMemcpyDeviceToHost, c->stream));
+    JCI_CHECK(hipMemcpyAsync(deg, d_dg, sizeof(int) * tot, hipMemcpyDeviceToHost, c->stream));
+    JCI_CHECK(hipMemcpyAsync(counts, d_cnt, sizeof(long long) * nrows, hipMemcpyDeviceToHost, c->stream));
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    hipFree(d_rows); hipFree(d_cols); hipFree(d_vals); hipFree(d_ph); hipFree(d_dg); hipFree(d_cnt);
+    return 0;
+}
+
+static int resolve_method(const jci_ham *h, int method, int want32)
+{
+    if (method != JCI_SIGMA_AUTO) return method;
+    if (want32) return h->d_val32 ? JCI_SIGMA_PAIRS32 : JCI_SIGMA_DIRECT32;
+    if (h->d_val64) return h->upper ? JCI_SIGMA_PAIRS_SYM64 : JCI_SIGMA_PAIRS64;
+    return JCI_SIGMA_DIRECT64;
+}
+
+static int ensure_xf(jci_ham *h, long long n)
+{
+    if (h->xf_len >= n) return 0;
+    hipFree(h->d_xf);
+    h->d_xf = nullptr;
+    h->xf_len = 0;
+    if (hipMalloc(&h->d_xf, sizeof(float) * n) != hipSuccess) {
+        jci_set_error("sigma: out of device memory for FP32 staging");
+        return -3;
+    }
+    h->xf_len = n;
+    return 0;
+}
+
+// y = H x on device, x and y determinant-major [N][nvec]
+static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *y)
+{
+    jci_space *s = h->sp;
+    jci_ctx *c = s->ctx;
+    long long N = s->h.ndets;
+    if (nvec < 1 || nvec > JCI_MAXK) { jci_set_error("sigma: nvec must be 1..%d", JCI_MAXK); return -1; }
+    method = resolve_method(h, method, 0);
+    switch (method) {
+    case JCI_SIGMA_PAIRS64:
+        if (!h->d_val64 || h->upper) { jci_set_error("sigma: no full FP64 pair list"); return -1; }
+        return jk_spmv(64, nvec, N, h->d_rowptr, h->d_col, h->d_val64, h->d_hd, x, y, c->stream);
+    case JCI_SIGMA_PAIRS32:
+        if (!h->d_val32 || h->upper) { jci_set_error("sigma: no full FP32 pair list"); return -1; }
+        RCHECK(ensure_xf(h, N * nvec));
+        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
+        return jk_spmv(32, nvec, N, h->d_rowptr, h->d_col, h->d_val32, h->d_hd, h->d_xf, y, c->stream);
+    case JCI_SIGMA_PAIRS_SYM64:
+        if (!h->d_val64 || !h->upper) { jci_set_error("sigma: no upper FP64 pair list"); return -1; }
+        if (nvec == 1)
+            return jk_spmv_sym(1, N, h->d_rowptr, h->d_col, h->d_val64, h->d_hd, x, y, c->stream);
+        jci_set_error("sigma: symmetric pair list takes one vector per call");
+        return -1;
+    case JCI_SIGMA_DIRECT64:
+        return jk_sigma_generic(64, nvec, s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom,
+                                h->d_m1, h->d_m2, h->d_hd, x, y, c->stream);
+    case JCI_SIGMA_DIRECT32:
+        RCHECK(ensure_xf(h, N * nvec));
+        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
+        return jk_sigma_generic(32, nvec, s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom,
+                                h->d_m1, h->d_m2, h->d_hd, h->d_xf, y, c->stream);
+    default:
+        jci_set_error("sigma: unknown method %d", method);
+        return -1;
+    }
+}
+
+JCI_API int jci_sigma_dev(jci_ham *h, int method, int nvec, const double *x_dev, double *y_dev)
+{
+    JCI_CHECK(hipSetDevice(h->sp->ctx->device));
+    return sigma_dev(h, method, nvec, x_dev, y_dev);
+}
+
+JCI_API int jci_sigma(jci_ham *h, int method, int nvec, const double *x, double *y)
+{
+    jci_ctx *c = h->sp->ctx;
+    JCI_CHECK(hipSetDevice(c->device));
+    long long N = h->sp->h.ndets;
+    double *dx, *dy;
+    JCI_CHECK(hipMalloc(&dx, sizeof(double) * N * nvec));
+    JCI_CHECK(hipMalloc(&dy, sizeof(double) * N * nvec));
+    JCI_CHECK(hipMemcpyAsync(dx, x, sizeof(double) * N * nvec, hipMemcpyHostToDevice, c->stream));
+    int rc = sigma_dev(h, method, nvec, dx, dy);
+    if (rc == 0) {
+        JCI_CHECK(hipMemcpyAsync(y, dy, sizeof(double) * N * nvec, hipMemcpyDeviceToHost, c->stream));
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+    }
+    hipFree(dx);
+    hipFree(dy);
+    return rc;
+}
+
+// ---- small dense eigensolver (cyclic Jacobi, FP64) ------------------------------------------
+// eigenvalues ascending; eigenvector i in vec[i*n + 0..n-1]
+static void jacobi_eig(int n, const double *a_in, double *eval, double *vec)
+{
+    std::vector<double> a(a_in, a_in + (size_t)n * n), v((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double x = a[(size_t)i * n + j] * a[(size_t)i * n + j];
+                tot += x;
+                if (i != j) off += x;
+            }
+        if (off <= 1e-30 * (tot > 0 ? tot : 1.0)) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double apq = a[(size_t)p * n + q];
+                if (fabs(apq) < 1e-300) continue;
+                double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < n; k++) {
+                    double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = cs * akp - sn * akq;
+                    a[(size_t)k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = cs * apk - sn * aqk;
+                    a[(size_t)q * n + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+                    v[(size_t)k * n + p] = cs * vkp - sn * vkq;
+                    v[(size_t)k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        return a[(size_t)x * n + x] < a[(size_t)y * n + y];
+    });
+    for (int i = 0; i < n; i++) {
+        eval[i] = a[(size_t)ord[i] * n + ord[i]];
+        // fix the sign: largest-magnitude component positive (deterministic vectors)
+        int arg = 0;
+        for (int k = 1; k < n; k++)
+            if (fabs(v[(size_t)k * n + ord[i]]) > fabs(v[(size_t)arg * n + ord[i]])) arg = k;
+        double sg = v[(size_t)arg * n + ord[i]] < 0 ? -1.0 : 1.0;
+        for (int k = 0; k < n; k++) vec[(size_t)i * n + k] = sg * v[(size_t)k * n + ord[i]];
+    }
+}
+
+JCI_API int jci_gram(jci_ctx *c, int m, int w, long long n, const double *V, const double *W,
+                     double *G)
+{
+    if (m < 1 || w < 1 || m > JCI_MAXBASIS || w > JCI_MAXBASIS || n < 1) {
+        jci_set_error("gram: dimensions out of range");
+        return -1;
+    }
+    JCI_CHECK(hipSetDevice(c->device));
+    double *dV, *dW;
+    JCI_CHECK(hipMalloc(&dV, sizeof(double) * m * n));
+    JCI_CHECK(hipMalloc(&dW, sizeof(double) * w * n));
+    JCI_CHECK(hipMemcpyAsync(dV, V, sizeof(double) * m * n, hipMemcpyHostToDevice, c->stream));
+    JCI_CHECK(hipMemcpyAsync(dW, W, sizeof(double) * w * n, hipMemcpyHostToDevice, c->stream));
+    int rc = jk_gram(m, w, n, dV, n, dW, n, c->d_partial, JCI_PARTIAL_BLOCKS, c->d_small, c->stream);
+    if (rc == 0) {
+        JCI_CHECK(hipMemcpyAsync(G, c->d_small, sizeof(double) * m * w, hipMemcpyDeviceToHost, c->stream));
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+    }
+    hipFree(dV);
+    hipFree(dW);
+    return rc;
+}
+
+JCI_API int jci_bench_sigma(jci_ham *h, int method, int nvec, int reps, double *ms)
+{
+    jci_ctx *c = h->sp->ctx;
+    JCI_CHECK(hipSetDevice(c->device));
+    long long N = h->sp->h.ndets;
+    if (reps < 1 || nvec < 1 || nvec > JCI_MAXK) { jci_set_error("bench_sigma: bad arguments"); return -1; }
+    double *x, *y;
+    JCI_CHECK(hipMalloc(&x, sizeof(double) * N * nvec));
+    JCI_CHECK(hipMalloc(&y, sizeof(double) * N * nvec));
+    int rc = jk_fill_random(N * nvec, 12345, x, c->stream);
+    if (!rc) rc = sigma_dev(h, method, nvec, x, y);       // warm-up launch
+    hipEvent_t e0, e1;
+    JCI_CHECK(hipEventCreate(&e0));
+    JCI_CHECK(hipEventCreate(&e1));
+    if (!rc) {
+        JCI_CHECK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps && !rc; r++) rc = sigma_dev(h, method, nvec, x, y);
+        JCI_CHECK(hipEventRecord(e1, c->stream));
+        JCI_CHECK(hipEventSynchronize(e1));
+        float t = 0.0f;
+        JCI_CHECK(hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(y);
+    return rc;
+}
+
+JCI_API int jci_bench_copy(jci_ctx *c, long long bytes, int reps, double *ms)
+{
+    JCI_CHECK(hipSetDevice(c->device));
+    if (reps < 1 || bytes < 32) { jci_set_error("bench_copy: bad arguments"); return -1; }
+    bytes = bytes / 32 * 32;
+    void *a, *b;
+    JCI_CHECK(hipMalloc(&a, bytes));
+    JCI_CHECK(hipMalloc(&b, bytes));
+    int rc = jk_fill_random(bytes / 8, 7, (double *)a, c->stream);
+    if (!rc) rc = jk_copy(bytes, a, b, c->stream);
+    hipEvent_t e0, e1;
+    JCI_CHECK(hipEventCreate(&e0));
+    JCI_CHECK(hipEventCreate(&e1));
+    if (!rc) {
+        JCI_CHECK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps && !rc; r++) rc = jk_copy(bytes, a, b, c->stream);
+        JCI_CHECK(hipEventRecord(e1, c->stream));
+        JCI_CHECK(hipEventSynchronize(e1));
+        float t = 0.0f;
+        JCI_CHECK(hipEventElapsedTime(&t, e0, e1));
+        *ms = (double)t / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
+JCI_API int jci_jacobi(int n, const double *a, double *eval, double *vec)
+{
+    if (n < 1) return -1;
+    jacobi_eig(n, a, eval, vec);
+    return 0;
+}
+
+// ---- Davidson ------------------------------------------------------------------------------
+JCI_API void jci_solve_defaults(jci_solve_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->maxiter = 200;
+    p->krymin = 4;
+    p->krymax = 16;
+    p->nroots = 1;
+    p->refdim = 64;
+    p->restol = 1e-6;
+    p->precision = JCI_PREC_FP64;
+    p->mixed_switch = 1e-4;
+    p->guess = JCI_GUESS_BLOCK;
+}
+
+struct Work {
+    SolveOp *op;
+    jci_ctx *c;
+    long long N, ld;
+    double *V, *S, *T, *X, *Y;   // V,S: krymax vectors; T: JCI_MAXK vectors; X,Y: [N][JCI_MAXK]
+    double sigma_time;
+    long long ns64, ns32;
+};
+
+// S[first .. first+cnt) = H V[first .. first+cnt), in chunks of JCI_MAXK
+static int sigma_block(Work &w, int first, int cnt, int method)
+{
+    double t0 = now_s();
+    for (int b = first; b < first + cnt; b += JCI_MAXK) {
+        int k = std::min(JCI_MAXK, first + cnt - b);
+        RCHECK(jk_pack(k, w.N, w.V + (size_t)b * w.ld, w.ld, w.X, w.c->stream));
+        RCHECK(w.op->sigma(w.op->self, method, k, w.X, w.Y));
+        RCHECK(jk_unpack(k, w.N, w.Y, w.S + (size_t)b * w.ld, w.ld, w.c->stream));
+    }
+    JCI_CHECK(hipStreamSynchronize(w.c->stream));
+    w.sigma_time += now_s() - t0;
+    return 0;
+}
+
+static int gram_host(Work &w, int m, const double *A, int n, const double *B, double *out)
+{
+    RCHECK(jk_gram(m, n, w.N, A, w.ld, B, w.ld, w.c->d_partial, JCI_PARTIAL_BLOCKS, w.c->d_small,
+                   w.c->stream));
+    JCI_CHECK(hipMemcpyAsync(w.c->h_small, w.c->d_small, sizeof(double) * m * n,
+                             hipMemcpyDeviceToHost, w.c->stream));
+    JCI_CHECK(hipStreamSynchronize(w.c->stream));
+    memcpy(out, w.c->h_small, sizeof(double) * m * n);
+    return 0;
+}
+
+// upload a small matrix to device scratch (after the device-side uses of d_small finished)
+static int upload_small(Work &w, const double *src, int n, double **dptr)
+{
+    JCI_CHECK(hipStreamSynchronize(w.c->stream));
+    memcpy(w.c->h_small, src, sizeof(double) * n);
+    JCI_CHECK(hipMemcpyAsync(w.c->d_small, w.c->h_small, sizeof(double) * n, hipMemcpyHostToDevice,
+                             w.c->stream));
+    *dptr = w.c->d_small;
+    return 0;
+}
+
+// orthonormalize T[0..k) against V[0..m) (two classical Gram-Schmidt passes) and among
+// themselves (symmetric orthonormalization, dropping dependent directions); returns the
+// number of vectors kept in *kept (moved to the front of T)
+static int orthonormalize_new(Work &w, int m, int k, int *kept)
+{
+    std::vector<double> cmat((size_t)m * k), nb((size_t)m * k);
+    double *dY;
+    for (int pass = 0; pass < 2 && m > 0; pass++) {
+        RCHECK(gram_host(w, m, w.V, k, w.T, cmat.data()));
+        for (size_t i = 0; i < cmat.size(); i++) nb[i] = -cmat[i];
+        RCHECK(upload_small(w, nb.data(), m * k, &dY));
+        RCHECK(jk_combine(m, k, w.N, w.V, w.ld, dY, k, w.T, w.ld, 1.0, w.c->stream));
+    }
+    std::vector<double> g((size_t)k * k), ev(k), U((size_t)k * k);
+    RCHECK(gram_host(w, k, w.T, k, w.T, g.data()));
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < i; j++) g[i * k + j] = g[j * k + i] = 0.5 * (g[i * k + j] + g[j * k + i]);
+    jacobi_eig(k, g.data(), ev.data(), U.data());
+    double emax = 0.0;
+    for (int i = 0; i < k; i++) emax = std::max(emax, ev[i]);
+    // keep directions with eigenvalue above a relative threshold, largest first
+    std::vector<double> Y;
+    int nk = 0;
+    for (int i = k - 1; i >= 0; i--) {
+        if (ev[i] <= 1e-12 * emax || ev[i] <= 1e-24) continue;
+        nk++;
+    }
+    Y.assign((size_t)k * nk, 0.0);
+    int col = 0;
+    for (int i = k - 1; i >= 0 && col < nk; i--) {
+        if (ev[i] <= 1e-12 * emax || ev[i] <= 1e-24) continue;
+        double s = 1.0 / sqrt(ev[i]);
+        for (int r = 0; r < k; r++) Y[(size_t)r * nk + col] = U[(size_t)i * k + r] * s;
+        col++;
+    }
+    if (nk > 0) {
+        RCHECK(upload_small(w, Y.data(), k * nk, &dY));
+        RCHECK(jk_rotate_inplace(k, nk, w.N, w.T, w.ld, dY, w.c->stream));
+    }
+    *kept = nk;
+    return 0;
+}
+
+static int guess_block(Work &w, const jci_solve_params *p, int nguess)
+{
+    SolveOp *op = w.op;
+    long long N = w.N;
+    int nb = (int)std::min<long long>(p->refdim, N);
+    if (nb < nguess) nb = (int)std::min<long long>(nguess, N);
+    std::vector<long long> idx(nb);
+    if (p->guess == JCI_GUESS_DIAG) {
+        std::vector<double> hd(N);
+        JCI_CHECK(hipMemcpyAsync(hd.data(), op->d_hd, sizeof(double) * N, hipMemcpyDeviceToHost,
+                                 w.c->stream));
+        JCI_CHECK(hipStreamSynchronize(w.c->stream));
+        std::vector<long long> ord(N);
+        for (long long i = 0; i < N; i++) ord[i] = i;
+        std::partial_sort(ord.begin(), ord.begin() + nb, ord.end(), [&](long long a, long long b) {
+            return hd[a] < hd[b] || (hd[a] == hd[b] && a < b);
+        });
+        for (int i = 0; i < nb; i++) idx[i] = ord[i];
+    } else {
+        for (int i = 0; i < nb; i++) idx[i] = i;
+    }
+    long long *d_idx;
+    uint64_t *d_m;
+    double *d_H;
+    JCI_CHECK(hipMalloc(&d_idx, sizeof(long long) * nb));
+    JCI_CHECK(hipMalloc(&d_m, sizeof(uint64_t) * 2 * nb));
+    JCI_CHECK(hipMalloc(&d_H, sizeof(double) * nb * nb));
+    JCI_CHECK(hipMemcpyAsync(d_idx, idx.data(), sizeof(long long) * nb, hipMemcpyHostToDevice,
+                             w.c->stream));
+    RCHECK(op->masks(op->self, nb, d_idx, d_m, d_m + nb));
+    RCHECK(jk_dense_block(nb, d_m, d_m + nb, 0, op->d_m1, op->d_m2, d_H, w.c->stream));
+    std::vector<double> H((size_t)nb * nb), ev(nb), U((size_t)nb * nb);
+    JCI_CHECK(hipMemcpyAsync(H.data(), d_H, sizeof(double) * nb * nb, hipMemcpyDeviceToHost,
+                             w.c->stream));
+    JCI_CHECK(hipStreamSynchronize(w.c->stream));
+    jacobi_eig(nb, H.data(), ev.data(), U.data());
+    // first nguess eigenvectors of the block, scattered to their determinants
+    double *d_U;
+    JCI_CHECK(hipMalloc(&d_U, sizeof(double) * nb * nguess));
+    JCI_CHECK(hipMemcpyAsync(d_U, U.data(), sizeof(double) * nb * nguess, hipMemcpyHostToDevice,
+                             w.c->stream));
+    JCI_CHECK(hipMemsetAsync(w.V, 0, sizeof(double) * w.ld * nguess, w.c->stream));
+    RCHECK(jk_scatter_guess(nb, nguess, d_idx, d_U, w.V, w.ld, w.c->stream));
+    JCI_CHECK(hipStreamSynchronize(w.c->stream));
+    hipFree(d_idx); hipFree(d_m); hipFree(d_H); hipFree(d_U);
+    return 0;
+}
+
+int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, double *evals,
+                   double *evecs, jci_solve_stats *stats)
+{
+    double t_start = now_s();
+    jci_ctx *c = op->ctx;
+    JCI_CHECK(hipSetDevice(c->device));
+    long long N = op->N;
+    int nroots = p->nroots, krymax = std::min<long long>(p->krymax, N);
+    int krymin = std::max(p->krymin, nroots);
+    if (nroots < 1 || nroots > JCI_MAXROOTS || nroots > N) { jci_set_error("solve: nroots out of range"); return -1; }
+    if (krymax > JCI_MAXBASIS) { jci_set_error("solve: krymax above %d", JCI_MAXBASIS); return -1; }
+    if (krymin > krymax) krymin = krymax;
+    if (krymin < nroots) { jci_set_error("solve: krymax below nroots"); return -1; }
+    Work w{};
+    w.op = op; w.c = c; w.N = N;
+    w.ld = (N + 63) / 64 * 64;
+    size_t vb = sizeof(double) * (size_t)w.ld;
+    if (hipMalloc(&w.V, vb * krymax) != hipSuccess || hipMalloc(&w.S, vb * krymax) != hipSuccess ||
+        hipMalloc(&w.T, vb * JCI_MAXK) != hipSuccess || hipMalloc(&w.X, vb * JCI_MAXK) != hipSuccess ||
+        hipMalloc(&w.Y, vb * JCI_MAXK) != hipSuccess) {
+        hipFree(w.V); hipFree(w.S); hipFree(w.T); hipFree(w.X); hipFree(w.Y);
+        jci_set_error("solve: out of device memory for %d basis vectors of %lld", krymax, N);
+        return -3;
+    }
+    int rc = 0;
+    int mixed = p->precision == JCI_PREC_MIXED;
+    int phase32 = mixed;
+    int m64 = op->resolve(op->self, p->sigma_fp64, 0), m32 = op->resolve(op->self, p->sigma_fp32, 1);
+    auto cur_method = [&]() { return phase32 ? m32 : m64; };
+    auto count_sigma = [&](int n) { if (phase32) w.ns32 += n; else w.ns64 += n; };
+    int m = 0, it = 0, converged = 0;
+    std::vector<double> G, theta(JCI_MAXBASIS), Yf, rn(JCI_MAXROOTS, 0.0);
+    double *dY;
+    // initial space
+    if (p->guess == JCI_GUESS_USER) {
+        if (!guess) { jci_set_error("solve: user guess requested without vectors"); rc = -1; goto done; }
+        for (int i = 0; i < krymin; i++) {
+            if (hipMemcpyAsync(w.T + (size_t)(i % JCI_MAXK) * w.ld, guess + (size_t)i * N,
+                               sizeof(double) * N, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+                rc = -2; goto done;
+            }
+            if (i % JCI_MAXK == JCI_MAXK - 1 || i == krymin - 1) {
+                int k = i % JCI_MAXK + 1, kept;
+                if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;
+                if (hipMemcpyAsync(w.V + (size_t)m * w.ld, w.T, vb * kept, hipMemcpyDeviceToDevice,
+                                   c->stream) != hipSuccess) { rc = -2; goto done; }
+                m += kept;
+            }
+        }
+    } else {
+        if ((rc = guess_block(w, p, krymin))) goto done;
+        m = krymin;
+    }
+    if (m < nroots) { jci_set_error("solve: initial space has fewer vectors than roots"); rc = -1; goto done; }
+    if ((rc = sigma_block(w, 0, m, cur_method()))) goto done;
+    count_sigma(m);
+    for (it = 1; it <= p->maxiter; it++) {
+        G.assign((size_t)m * m, 0.0);
+        if ((rc = gram_host(w, m, w.V, m, w.S, G.data()))) goto done;
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < i; j++) G[i * m + j] = G[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
+        Yf.assign((size_t)m * m, 0.0);
+        jacobi_eig(m, G.data(), theta.data(), Yf.data());
+        // residuals of the lowest nroots Ritz pairs -> T (preconditioned), norms
+        std::vector<double> Ysel((size_t)m * nroots);
+        for (int i = 0; i < m; i++)
+            for (int r = 0; r < nroots; r++) Ysel[(size_t)i * nroots + r] = Yf[(size_t)r * m + i];
+        double *d_th;
+        // Ysel and theta share the small scratch: Ysel at [0, m*nroots), theta after it
+        std::vector<double> packed(Ysel);
+        packed.insert(packed.end(), theta.begin(), theta.begin() + nroots);
+        if ((rc = upload_small(w, packed.data(), (int)packed.size(), &dY))) goto done;
+        d_th = dY + (size_t)m * nroots;
+        if ((rc = jk_residual(m, nroots, N, w.V, w.S, w.ld, dY, d_th, op->d_hd, w.T,
+                              c->d_partial, JCI_PARTIAL_BLOCKS, c->d_small + 2048, c->stream)))
+            goto done;
+        if (hipMemcpyAsync(c->h_small, c->d_small + 2048, sizeof(double) * nroots,
+                           hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { rc = -2; goto done; }
+        double rmax = 0.0;
+        for (int r = 0; r < nroots; r++) { rn[r] = sqrt(c->h_small[r]); rmax = std::max(rmax, rn[r]); }
+        if (p->verbose)
+            fprintf(stdout, " iter %3d  m %2d  %s  E0 %.12f  max|r| %.3e\n", it, m,
+                    phase32 ? "fp32" : "fp64", theta[0] + p->totfrze, rmax);
+        if (phase32 && (rmax < p->mixed_switch || it == p->maxiter)) {
+            // hand over to FP64: recompute sigma of the whole basis in FP64
+            phase32 = 0;
+            if ((rc = sigma_block(w, 0, m, cur_method()))) goto done;
+            count_sigma(m);
+            continue;
+        }
+        std::vector<int> todo;
+        for (int r = 0; r < nroots; r++)
+            if (!(rn[r] < p->restol)) todo.push_back(r);
+        if (todo.empty()) { converged = 1; break; }
+        if (it == p->maxiter) break;
+        int k = (int)todo.size();
+        // compact the unconverged corrections to the front of T
+        for (int t = 0; t < k; t++)
+            if (todo[t] != t &&
+                hipMemcpyAsync(w.T + (size_t)t * w.ld, w.T + (size_t)todo[t] * w.ld, vb,
+                               hipMemcpyDeviceToDevice, c->stream) != hipSuccess) { rc = -2; goto done; }
+        if (m + k > krymax) {
+            // collapse onto the lowest kc Ritz vectors (basis and sigma rotated together)
+            int kc = std::max(krymin, nroots);
+            kc = std::min(kc, krymax - k);
+            if (kc < nroots) kc = nroots;
+            std::vector<double> Yc((size_t)m * kc);
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < kc; j++) Yc[(size_t)i * kc + j] = Yf[(size_t)j * m + i];
+            if ((rc = upload_small(w, Yc.data(), m * kc, &dY))) goto done;
+            if ((rc = jk_rotate_inplace(m, kc, N, w.V, w.ld, dY, c->stream))) goto done;
+            if ((rc = jk_rotate_inplace(m, kc, N, w.S, w.ld, dY, c->stream))) goto done;
+            m = kc;
+        }
+        int kept = 0;
+        if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;
+        if (kept == 0) break;   // no new directions: stagnated
+        if (hipMemcpyAsync(w.V + (size_t)m * w.ld, w.T, vb * kept, hipMemcpyDeviceToDevice,
+                           c->stream) != hipSuccess) { rc = -2; goto done; }
+        if ((rc = sigma_block(w, m, kept, cur_method()))) goto done;
+        count_sigma(kept);
+        m += kept;
+    }
+    for (int r = 0; r < nroots; r++) evals[r] = theta[r] + p->totfrze;
+    if (evecs) {
+        std::vector<double> Yc((size_t)m * nroots);
+        for (int i = 0; i < m; i++)
+            for (int r = 0; r < nroots; r++) Yc[(size_t)i * nroots + r] = Yf[(size_t)r * m + i];
+        if ((rc = upload_small(w, Yc.data(), m * nroots, &dY))) goto done;
+        if ((rc = jk_combine(m, nroots, N, w.V, w.ld, dY, nroots, w.T, w.ld, 0.0, c->stream))) goto done;
+        for (int r = 0; r < nroots; r++)
+            if (hipMemcpyAsync(evecs + (size_t)r * N, w.T + (size_t)r * w.ld, sizeof(double) * N,
+                               hipMemcpyDeviceToHost, c->stream) != hipSuccess) { rc = -2; goto done; }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = -2; goto done; }
+    }
+    if (!converged && rc == 0) {
+        jci_set_error("solve: not converged in %d iterations (max|r| above %.3e)", p->maxiter, p->restol);
+        rc = 1;
+    }
+done:
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->iterations = it;
+        stats->converged = converged;
+        stats->nbasis = m;
+        stats->nsigma64 = w.ns64;
+        stats->nsigma32 = w.ns32;
+        for (int r = 0; r < nroots && r < JCI_MAXROOTS; r++) stats->rnorm[r] = rn[r];
+        stats->sigma_seconds = w.sigma_time;
+        stats->seconds = now_s() - t_start;
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(w.V); hipFree(w.S); hipFree(w.T); hipFree(w.X); hipFree(w.Y);
+    return rc;
+}
+
+static int ham_sigma(void *self, int method, int k, const double *x, double *y)
+{
+    return sigma_dev((jci_ham *)self, method, k, x, y);
+}
+
+static int ham_masks(void *self, long long n, const long long *d_idx, uint64_t *oa, uint64_t *ob)
+{
+    jci_ham *h = (jci_ham *)self;
+    return jk_det_masks(h->sp->d_desc, h->sp->d_mask[0], h->sp->d_mask[1], d_idx, 0, n, oa, ob,
+                        h->sp->ctx->stream);
+}
+
+static int ham_resolve(void *self, int method, int want32)
+{
+    return resolve_method((jci_ham *)self, method, want32);
+}
+
+JCI_API int jci_solve(jci_ham *h, const jci_solve_params *p, const double *guess,
+                         double *evals, double *evecs, jci_solve_stats *stats)
+{
+    SolveOp op;
+    op.self = h;
+    op.ctx = h->sp->ctx;
+    op.N = h->sp->h.ndets;
+    op.d_hd = h->d_hd;
+    op.d_m1 = h->d_m1;
+    op.d_m2 = h->d_m2;
+    op.sigma = ham_sigma;
+    op.masks = ham_masks;
+    op.resolve = ham_resolve;
+    return jci_solve_core(&op, p, guess, evals, evecs, stats);
+}
+
+// ---- synthetic integrals ------------------------------------------------------------------
+// SplitMix64 (Steele, Lea, Flood 2014) drives a uniform stream; normals by Box-Muller
+// (cosine branch only, one normal per two uniforms).  Draw order: h_ij for i < j
+// (row-major over 1-based i, j), then for P = 1..2n the upper triangle of L^P (row-major,
+// diagonal first in each row).
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+    double normal()
+    {
+        double u1 = 1.0 - uniform(), u2 = uniform();
+        return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+};
+
+JCI_API int jci_synth_integrals(int n, uint64_t seed, double *m1, double *m2)
+{
+    if (n < 1 || n > 64) { jci_set_error("synth: norbs out of range"); return -1; }
+    SplitMix rng{seed};
+    size_t n1 = m1_len(n);
+    std::vector<double> h((size_t)n * n, 0.0);
+    for (int i = 1; i <= n; i++) h[(size_t)(i - 1) * n + (i - 1)] = -2.0 + 0.25 * i;
+    for (int i = 1; i <= n; i++)
+        for (int j = i + 1; j <= n; j++) {
+            double v = 0.05 * rng.normal();
+            h[(size_t)(i - 1) * n + (j - 1)] = h[(size_t)(j - 1) * n + (i - 1)] = v;
+        }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j <= i; j++) m1[jci_tri(i, j)] = h[(size_t)i * n + j];
+    // L[P][pair] with pair = tri(i, j)
+    int np = 2 * n;
+    std::vector<double> L((size_t)np * n1);
+    for (int P = 0; P < np; P++)
+        for (int i = 0; i < n; i++)
+            for (int j = i; j < n; j++) {
+                double v = (i == j) ? 0.1 + 0.2 * rng.uniform() : 0.02 * rng.normal();
+                L[(size_t)P * n1 + jci_tri(i, j)] = v;
+            }
+    for (size_t a = 0; a < n1; a++)
+        for (size_t b = 0; b <= a; b++) {
+            double s = 0.0;
+            for (int P = 0; P < np; P++) s += L[(size_t)P * n1 + a] * L[(size_t)P * n1 + b];
+            m2[a * (a + 1) / 2 + b] = s;
+        }
+    return 0;
+}

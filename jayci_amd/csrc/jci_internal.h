@@ -1,0 +1,133 @@
+// Internal declarations shared by the host code and the HIP kernels of libjayci_gpu.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define JCI_MAXG 16          // electron-occupation groups per spin
+#define JCI_MAXB (JCI_MAXG * JCI_MAXG)
+#define JCI_MAXBASIS 64      // Davidson subspace limit (krymax)
+#define JCI_MAXK 8           // vectors per sigma launch
+
+#define JCI_CHECK(x)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            jci_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+            return -2;                                                                 \
+        }                                                                              \
+    } while (0)
+
+void jci_set_error(const char *fmt, ...);
+
+// One spin's string set.  Strings of group g occupy [gstart[g], gstart[g]+gn[g]); inside a
+// group the composite rank is (r_docc * nsa + r_actv) * nsv + r_virt, each r the
+// lexicographic rank of the sub-combination inside its orbital subspace.
+struct SpinDesc {
+    int nelec, ngrp;
+    int gd[JCI_MAXG], ga[JCI_MAXG], gv[JCI_MAXG];
+    long long gstart[JCI_MAXG + 1], gn[JCI_MAXG], nsa[JCI_MAXG], nsv[JCI_MAXG];
+};
+
+// Whole determinant space: blocks (p,q) in reference order, alpha-major inside a block.
+struct SpaceDesc {
+    int norbs, ndocc, nactv, nvirt;
+    unsigned long long dmask, amask, vmask, allmask;
+    SpinDesc s[2];
+    int nblk;
+    int blkid[JCI_MAXG][JCI_MAXG];
+    int bp[JCI_MAXB], bq[JCI_MAXB];
+    long long boff[JCI_MAXB + 1];
+    long long ndets;
+};
+
+// host-side construction (space.cpp)
+int jci_build_desc(int na, int nb, int norbs, int ndocc, int nactv, int xlvl, int rule,
+                   SpaceDesc *d);
+void jci_binom_table(unsigned long long *tab /* 65*65 */);
+
+#define JCI_API extern "C" __attribute__((visibility("default")))
+
+// ---- device context (jci.cpp) ------------------------------------------------------------
+#define JCI_PARTIAL_BLOCKS 512
+struct jci_ctx {
+    int device;
+    int refs;                   // spaces holding this context
+    int closing;                // destroy requested while spaces were alive
+    hipStream_t stream;
+    double *d_partial;          // JCI_PARTIAL_BLOCKS * 64 * 64 block partials
+    double *d_small;            // 64 * 64 small operands / results
+    unsigned long long *d_binom;
+    double *h_small;            // pinned mirror of d_small
+};
+
+// ---- Davidson over an abstract operator (jci.cpp) -------------------------------------
+struct SolveOp {
+    void *self;
+    jci_ctx *ctx;
+    long long N;
+    const double *d_hd, *d_m1, *d_m2;
+    // y = H x, determinant-major [N][k] device arrays
+    int (*sigma)(void *self, int method, int k, const double *x, double *y);
+    // masks of determinants d_idx[0..n) into device arrays oa, ob
+    int (*masks)(void *self, long long n, const long long *d_idx, uint64_t *oa, uint64_t *ob);
+    // concrete sigma method for a requested one (0 = automatic) and precision
+    int (*resolve)(void *self, int method, int want32);
+};
+#include "../../include/jayci_gpu.h"
+int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, double *evals,
+                   double *evecs, jci_solve_stats *stats);
+
+// ---- kernel launchers (kernels.hip) ------------------------------------------------
+int jk_gen_strings(const SpaceDesc *d_desc, const SpaceDesc &h, int spin,
+                   const unsigned long long *d_binom, uint64_t *d_mask, hipStream_t st);
+int jk_hdiag(const SpaceDesc *d_desc, long long ndets, const uint64_t *ma, const uint64_t *mb,
+             const double *m1, const double *m2, double *hd, hipStream_t st);
+int jk_row_count(const SpaceDesc *d_desc, long long ndets, const uint64_t *ma,
+                 const uint64_t *mb, const unsigned long long *binom, long long *cnt,
+                 int upper, hipStream_t st);
+int jk_row_fill(const SpaceDesc *d_desc, long long ndets, const uint64_t *ma, const uint64_t *mb,
+                const unsigned long long *binom, const double *m1, const double *m2,
+                const long long *rowptr, int *col, double *val64, float *val32, int upper,
+                hipStream_t st);
+int jk_rows_debug(const SpaceDesc *d_desc, const uint64_t *ma, const uint64_t *mb,
+                  const unsigned long long *binom, const double *m1, const double *m2,
+                  const long long *rows, int nrows, long long cap, long long *cols, double *vals,
+                  int *phase, int *deg, long long *counts, hipStream_t st);
+int jk_scan(const long long *cnt, long long *rowptr, long long n, hipStream_t st);
+int jk_spmv(int prec, int nvec, long long N, const long long *rowptr, const int *col,
+            const void *val, const double *hd, const void *x, double *y, hipStream_t st);
+int jk_spmv_sym(int nvec, long long N, const long long *rowptr, const int *col,
+                const double *val, const double *hd, const double *x, double *y, hipStream_t st);
+int jk_sigma_generic(int prec, int nvec, const SpaceDesc *d_desc, long long ndets,
+                     const uint64_t *ma, const uint64_t *mb, const unsigned long long *binom,
+                     const double *m1, const double *m2, const double *hd, const void *x,
+                     double *y, hipStream_t st);
+int jk_to_f32(const double *x, float *y, long long n, hipStream_t st);
+
+// Davidson vector kernels
+int jk_gram(int m, int w, long long N, const double *V, long long ldv, const double *W,
+            long long ldw, double *partial, int nblocks, double *out_dev, hipStream_t st);
+int jk_combine(int m, int k, long long N, const double *V, long long ldv, const double *Y,
+               int ldy, double *out, long long ldo, double beta, hipStream_t st);
+int jk_residual(int m, int k, long long N, const double *V, const double *S, long long ld,
+                const double *Y, const double *theta, const double *hd, double *T,
+                double *partial, int nblocks, double *norms_dev, hipStream_t st);
+int jk_pack(int k, long long N, const double *src, long long lds, double *dst, hipStream_t st);
+int jk_unpack(int k, long long N, const double *src, double *dst, long long ldd, hipStream_t st);
+int jk_colnorms(int k, long long N, const double *X, long long ld, double *partial,
+                int nblocks, double *out_dev, hipStream_t st);
+int jk_scale_cols(int k, long long N, double *X, long long ld, const double *s, hipStream_t st);
+int jk_dense_block(int nb, const uint64_t *ma_rows, const uint64_t *mb_rows, int norbs,
+                   const double *m1, const double *m2, double *H, hipStream_t st);
+
+// drop-in path on explicit determinant lists
+int jk_list_hdiag(long long n, const uint64_t *ma, const uint64_t *mb, int norbs,
+                  const double *m1, const double *m2, double *hd, hipStream_t st);
+int jk_list_screen_count(long long n, const uint64_t *ma, const uint64_t *mb, long long *cnt,
+                         hipStream_t st);
+int jk_list_screen_fill(long long n, const uint64_t *ma, const uint64_t *mb,
+                        const long long *rowptr, int *col, hipStream_t st);
+int jk_list_values(long long nnz_rows, long long n, const uint64_t *ma, const uint64_t *mb,
+                   int norbs, const double *m1, const double *m2, const long long *rowptr,
+                   const int *col, double *val, hipStream_t st);

@@ -1,0 +1,161 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same inputs.
+
+Integer work (strings, determinant order, connected pairs, phases, degrees) is compared
+bit for bit; FP64 matrix elements and sigma to 1e-12 relative; FP32 sigma to 2e-5
+relative of the vector scale; eigenvalues to 1e-8 Ha (restol 1e-7 gives errors ~1e-13).
+"""
+import numpy as np
+import pytest
+
+from conftest import SMALL_SPACES, rand_vectors
+
+pytestmark = pytest.mark.gpu
+
+
+def _spaces(J, orc, ctx, spec, seed=11):
+    na, nb, n, nd, nact, xl, rule = spec
+    m1, m2 = J.synth_integrals(n, seed)
+    gs = J.Space(ctx, na, nb, n, nd, nact, xl, rule)
+    os_ = orc.Space(na, nb, n, nd, nact, xl, rule)
+    return gs, os_, m1, m2
+
+
+@pytest.mark.parametrize("spec", SMALL_SPACES)
+def test_strings_groups_blocks_dets(J, orc, ctx, spec):
+    gs, os_, _, _ = _spaces(J, orc, ctx, spec)
+    assert gs.ndets == os_.ndets
+    for spin in (0, 1):
+        assert np.array_equal(gs.strings(spin), os_.strings(spin))
+        assert np.array_equal(gs.groups(spin), os_.groups(spin))
+    assert np.array_equal(gs.blocks(), os_.blocks())
+    a, b = gs.dets()
+    oa, ob = os_.dets()
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    # addressing round trip on a sample
+    rng = np.random.default_rng(0)
+    for I in rng.integers(0, gs.ndets, size=min(20, gs.ndets)):
+        assert gs.index(a[I], b[I]) == I
+
+
+@pytest.mark.parametrize("spec", SMALL_SPACES)
+def test_hdiag(J, orc, ctx, spec):
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec)
+    h = J.Hamiltonian(gs, m1, m2)
+    np.testing.assert_allclose(h.hdiag(), os_.hdiag(m1, m2), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("spec", SMALL_SPACES)
+def test_rows_bitexact(J, orc, ctx, spec):
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec)
+    h = J.Hamiltonian(gs, m1, m2)
+    N = gs.ndets
+    rows = np.unique(np.r_[0, N - 1, np.random.default_rng(1).integers(0, N, size=min(40, N))])
+    got = h.rows(rows)
+    for I, (c, v, ph, dg) in zip(rows, got):
+        oc, ov, oph, odg = os_.row(int(I), m1, m2)
+        assert np.array_equal(c, oc), "columns differ in row %d" % I
+        assert np.array_equal(ph, oph), "phases differ in row %d" % I
+        assert np.array_equal(dg, odg), "degrees differ in row %d" % I
+        np.testing.assert_allclose(v, ov, rtol=1e-12, atol=1e-13)
+
+
+METHODS64 = ["SIGMA_PAIRS64", "SIGMA_PAIRS_SYM64", "SIGMA_DIRECT64"]
+METHODS32 = ["SIGMA_PAIRS32", "SIGMA_DIRECT32"]
+
+
+@pytest.mark.parametrize("spec", SMALL_SPACES)
+def test_sigma_all_methods(J, orc, ctx, spec):
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec)
+    full = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32)
+    upper = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64 | J.PAIRS_UPPER)
+    N = gs.ndets
+    X = rand_vectors(N, 3, 5)
+    ref = os_.sigma(m1, m2, X)
+    scale = np.abs(ref).max()
+    for name in METHODS64:
+        meth = getattr(J, name)
+        h = upper if name == "SIGMA_PAIRS_SYM64" else full
+        k = 1 if name == "SIGMA_PAIRS_SYM64" else 3
+        y = h.sigma(X[:k], method=meth)
+        np.testing.assert_allclose(y, ref[:k], rtol=0, atol=1e-12 * scale, err_msg=name)
+    for name in METHODS32:
+        y = full.sigma(X, method=getattr(J, name))
+        np.testing.assert_allclose(y, ref, rtol=0, atol=2e-5 * scale, err_msg=name)
+
+
+def test_pair_counts_fci(J, orc, ctx):
+    # FCI 3a3b/8: connections per row = 2*3*5 singles + 2*C(3,2)*C(5,2) doubles + (3*5)^2
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, (3, 3, 8, 0, 8, 0, 1))
+    h = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64)
+    per_row = 2 * 15 + 2 * 3 * 10 + 15 * 15
+    assert h.npairs == gs.ndets * per_row
+    hu = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64 | J.PAIRS_UPPER)
+    assert hu.npairs == gs.ndets * per_row // 2 == os_.count_upper()
+
+
+@pytest.mark.parametrize("spec", [s for s in SMALL_SPACES if s[0] + s[1] > 1])
+def test_davidson_vs_dense(J, orc, ctx, spec):
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec)
+    N = gs.ndets
+    if N > 3200:
+        pytest.skip("dense reference too large")
+    H = os_.dense_h(m1, m2)
+    w = np.linalg.eigh(H)[0]
+    nroots = min(3, N - 1)
+    h = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32)
+    ev, vec, st = h.solve(nroots=nroots, restol=1e-7, krymin=nroots + 1, krymax=16,
+                          refdim=min(32, N))
+    np.testing.assert_allclose(ev, w[:nroots], rtol=0, atol=1e-8)
+    for r in range(nroots):
+        res = H @ vec[r] - ev[r] * vec[r]
+        assert np.linalg.norm(res) < 1e-6
+        assert abs(np.linalg.norm(vec[r]) - 1.0) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["PREC_FP64", "PREC_MIXED"])
+@pytest.mark.parametrize("flags,sig64,sig32", [
+    ("pairs", "SIGMA_PAIRS64", "SIGMA_PAIRS32"),
+    ("none", "SIGMA_DIRECT64", "SIGMA_DIRECT32"),
+])
+def test_davidson_precision_and_methods(J, orc, ctx, precision, flags, sig64, sig32):
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, (3, 3, 8, 2, 2, 2, 1), seed=3)
+    H = os_.dense_h(m1, m2)
+    w = np.linalg.eigh(H)[0]
+    fl = J.PAIRS_FP64 | J.PAIRS_FP32 if flags == "pairs" else 0
+    h = J.Hamiltonian(gs, m1, m2, fl)
+    ev, vec, st = h.solve(nroots=3, restol=1e-7, krymin=4, krymax=12, refdim=40,
+                          precision=getattr(J, precision), sigma_fp64=getattr(J, sig64),
+                          sigma_fp32=getattr(J, sig32), guess=J.GUESS_DIAG)
+    np.testing.assert_allclose(ev, w[:3], rtol=0, atol=1e-8)
+    if precision == "PREC_MIXED":
+        assert st["nsigma32"] > 0 and st["nsigma64"] > 0
+
+
+def test_user_guess_and_maxiter(J, orc, ctx):
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, (3, 3, 8, 0, 8, 0, 1), seed=4)
+    h = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64)
+    ev0, vec0, _ = h.solve(nroots=2, restol=1e-8, krymin=3, krymax=12)
+    # restart from the converged vectors: converges at once
+    g = np.vstack([vec0, rand_vectors(gs.ndets, 1, 9)])
+    ev1, _, st = h.solve(nroots=2, restol=1e-8, krymin=3, krymax=12, guess=J.GUESS_USER,
+                         guess_vectors=g)
+    np.testing.assert_allclose(ev1, ev0, atol=1e-10)
+    assert st["iterations"] <= 2
+    # maxiter exhausted: nonzero code 1, no exception with check=False
+    _, _, st2 = h.solve(nroots=2, restol=1e-14, krymin=3, krymax=6, maxiter=3, check=False)
+    assert st2["rc"] == 1 and not st2["converged"]
+
+
+def test_gram_mfma_layout(J, orc, ctx):
+    """The subspace matrix V^T (H V) goes through the MFMA Gram kernel; a 1-root solve on a
+    space whose H is diagonal-dominant exercises it, and the Rayleigh quotient of the
+    returned vector must equal the eigenvalue."""
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, (4, 4, 10, 2, 4, 2, 1), seed=6)
+    h = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64)
+    ev, vec, st = h.solve(nroots=5, restol=1e-7, krymin=8, krymax=40, refdim=64,
+                          guess=J.GUESS_DIAG)
+    hv = h.sigma(vec)
+    for r in range(5):
+        assert abs(vec[r] @ hv[r] - ev[r]) < 1e-10
+        for q in range(r):
+            assert abs(vec[r] @ vec[q]) < 1e-10
