@@ -26,6 +26,9 @@ PEAK_HBM = 8.0e12        # MI355X HBM3E, bytes/s
 
 from jayci_amd.workloads import WORKLOADS  # noqa: E402  (plain dict module)
 
+METHOD_NAMES = {1: "pairs64", 2: "pairs32", 3: "pairs_sym64", 4: "direct64", 5: "direct32",
+                6: "fast64", 7: "fast32"}
+
 
 def percentile(xs, q):
     xs = sorted(xs)
@@ -102,8 +105,10 @@ def main():
     sp = J.Space(ctx, wl["na"], wl["nb"], wl["norbs"], wl["ndocc"], wl["nactv"], wl["xlvl"],
                  wl["rule"])
     t0 = time.time()
-    h = J.Hamiltonian(sp, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32)
+    h = J.Hamiltonian(sp, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32 | J.DIRECT_TABLES)
     setup_s = time.time() - t0
+    # autotuner: fastest sigma variant / tile width for this space becomes the default
+    best64, best32, tune_ms, width = h.autotune(nvec=1, reps=5)
     N = sp.ndets
     gold = golden_energy(args.workload)
     solve_kw = dict(nroots=wl["nroots"], restol=1e-6, krymin=4, krymax=16, refdim=64,
@@ -112,7 +117,8 @@ def main():
     def log(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
 
-    log("setup %.2f s, N=%d, pairs=%d" % (setup_s, N, h.npairs))
+    log("setup %.2f s, N=%d, pairs=%d, autotune %s -> fp64 %d fp32 %d (W=%d)"
+        % (setup_s, N, h.npairs, tune_ms, best64, best32, width))
 
     def run(precision, k):
         lat, errs, stats = [], [], []
@@ -170,7 +176,10 @@ def main():
         "dtype": "f32 sigma -> f64 refinement (mixed); f64 (plain)",
         "data": "synthetic integrals (SplitMix64/Box-Muller, seed %d)" % wl["seed"],
         "config": {"workload": wl["desc"], "ndets": N, "pairs_full": P_full,
-                   "setup_s": round(setup_s, 3)},
+                   "setup_s": round(setup_s, 3),
+                   "sigma_fp64": METHOD_NAMES.get(best64), "sigma_fp32": METHOD_NAMES.get(best32),
+                   "tile_width": width,
+                   "sigma_ms": {METHOD_NAMES[m]: round(v, 4) for m, v in tune_ms.items()}},
         "mixed": {"solves_per_s": args.steps / tot_m, "p50_s": percentile(lat_m, 0.5),
                   "p99_s": percentile(lat_m, 0.99),
                   "max_abs_err_Ha": max(err_m) if err_m else None,

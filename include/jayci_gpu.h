@@ -139,6 +139,7 @@ typedef struct jci_ham jci_ham;
 #define JCI_PAIRS_FP64 1    /* keep the connected-pair list with FP64 values */
 #define JCI_PAIRS_FP32 2    /* ... and/or with FP32 values */
 #define JCI_PAIRS_UPPER 4   /* store only j > i (symmetric sigma with atomics) */
+#define JCI_DIRECT_TABLES 8 /* per-string excitation tables for the matrix-free sigma */
 
 #define JCI_SIGMA_AUTO 0
 #define JCI_SIGMA_PAIRS64 1
@@ -146,6 +147,8 @@ typedef struct jci_ham jci_ham;
 #define JCI_SIGMA_PAIRS_SYM64 3
 #define JCI_SIGMA_DIRECT64 4
 #define JCI_SIGMA_DIRECT32 5
+#define JCI_SIGMA_FAST64 6  /* matrix-free, excitation tables, FP64 */
+#define JCI_SIGMA_FAST32 7  /* matrix-free, excitation tables, FP32 accumulation */
 
 #define JCI_PREC_FP64 0
 #define JCI_PREC_MIXED 1
@@ -232,6 +235,14 @@ int jci_gram(jci_ctx *ctx, int m, int w, long long n, const double *V, const dou
  * probe and the byte-counter calibration of a profiler run). */
 int jci_bench_sigma(jci_ham *h, int method, int nvec, int reps, double *ms_per_launch);
 int jci_bench_copy(jci_ctx *ctx, long long bytes, int reps, double *ms_per_launch);
+
+/* Autotuner: times every sigma variant the Hamiltonian holds (pair list FP64/FP32;
+ * matrix-free FP64/FP32 with beta tiles of 64, 128 and 256 strings) on nvec vectors and
+ * makes the fastest FP64 / FP32 ones what JCI_SIGMA_AUTO resolves to.  times (may be
+ * NULL) gets ms per launch indexed by JCI_SIGMA_* (0 where not available). */
+int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32, double *times);
+/* beta strings per tile of the matrix-free sigma (0 without excitation tables) */
+int jci_ham_fast_width(const jci_ham *h);
 
 /* dense symmetric eigensolver (cyclic Jacobi, FP64) used for the Davidson subspace:
  * eigenvalues ascending, eigenvector i in vec[i*n + 0..n-1] (largest component positive) */

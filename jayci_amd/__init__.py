@@ -16,8 +16,9 @@ LIB_PATH = os.path.join(HERE, "libjayci_gpu.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "jayci_gpu.h")
 
 RULE_SERIAL, RULE_MRCISD = 0, 1
-PAIRS_FP64, PAIRS_FP32, PAIRS_UPPER = 1, 2, 4
-SIGMA_AUTO, SIGMA_PAIRS64, SIGMA_PAIRS32, SIGMA_PAIRS_SYM64, SIGMA_DIRECT64, SIGMA_DIRECT32 = range(6)
+PAIRS_FP64, PAIRS_FP32, PAIRS_UPPER, DIRECT_TABLES = 1, 2, 4, 8
+(SIGMA_AUTO, SIGMA_PAIRS64, SIGMA_PAIRS32, SIGMA_PAIRS_SYM64, SIGMA_DIRECT64, SIGMA_DIRECT32,
+ SIGMA_FAST64, SIGMA_FAST32) = range(8)
 PREC_FP64, PREC_MIXED = 0, 1
 GUESS_BLOCK, GUESS_USER, GUESS_DIAG = 1, 2, 3
 MAXROOTS = 8
@@ -81,6 +82,8 @@ _sig("jci_jacobi", _i, [_i, _P, _P, _P])
 _sig("jci_gram", _i, [_P, _i, _i, _ll, _P, _P, _P])
 _sig("jci_bench_sigma", _i, [_P, _i, _i, _i, ctypes.POINTER(_d)])
 _sig("jci_bench_copy", _i, [_P, _ll, _i, ctypes.POINTER(_d)])
+_sig("jci_ham_autotune", _i, [_P, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _P])
+_sig("jci_ham_fast_width", _i, [_P])
 
 
 class JciError(RuntimeError):
@@ -308,6 +311,15 @@ class Hamiltonian:
         ms = _d()
         _check(_L.jci_bench_sigma(self._h, method, nvec, reps, ctypes.byref(ms)), "jci_bench_sigma")
         return ms.value
+
+    def autotune(self, nvec=1, reps=5):
+        """Time all sigma variants; returns (best64, best32, {method: ms}, tile width)."""
+        b64, b32 = _i(), _i()
+        t = np.zeros(8)
+        _check(_L.jci_ham_autotune(self._h, nvec, reps, ctypes.byref(b64), ctypes.byref(b32), _p(t)),
+               "jci_ham_autotune")
+        return b64.value, b32.value, {m: t[m] for m in range(8) if t[m] > 0}, \
+            _L.jci_ham_fast_width(self._h)
 
     def sigma(self, x, method=SIGMA_AUTO):
         """H x for x of shape (N,) or (k, N) (vector-major); returns the same shape."""

@@ -2,6 +2,7 @@
 // diagonal, optional pair list), sigma dispatch and the block Davidson driver.
 #include "jci_internal.h"
 #include "slater.h"
+#include "sigma_fast.h"
 #include "../../include/jayci_gpu.h"
 
 #include <math.h>
@@ -248,6 +249,17 @@ struct jci_ham {
     int upper;
     float *d_xf;             // FP32 staging for sigma inputs
     long long xf_len;
+    // excitation tables of the matrix-free sigma (spin 1 aliases spin 0 when equal)
+    int have_fast;
+    long long *d_off1[2], *d_off2[2];
+    int2 *d_e1[2], *d_e2[2];
+    double *d_v1[2], *d_v2[2];
+    long long nexc1[2], nexc2[2];
+    double *d_m2sq;
+    long long *d_tile_off[3];   // tile maps for W = 64, 128, 256
+    FastArgs fast[3];
+    int fast_w;                 // index into fast[] used by the FAST methods
+    int tuned64, tuned32;       // autotuned default methods (0 = not tuned)
 };
 
 static size_t m1_len(int n) { return (size_t)n * (n + 1) / 2; }
@@ -292,6 +304,82 @@ static int build_pairs(jci_ham *h, int flags)
     return 0;
 }
 
+// excitation tables + tile map of the matrix-free sigma
+static int build_fast(jci_ham *h, int W)
+{
+    jci_space *s = h->sp;
+    jci_ctx *c = s->ctx;
+    const SpaceDesc &D = s->h;
+    bool same = s->na == s->nb;
+    for (int spin = 0; spin < (same ? 1 : 2); spin++) {
+        long long n = D.s[spin].gstart[D.s[spin].ngrp];
+        long long *c1, *c2;
+        JCI_CHECK(hipMalloc(&c1, sizeof(long long) * (n + 1)));
+        JCI_CHECK(hipMalloc(&c2, sizeof(long long) * (n + 1)));
+        JCI_CHECK(hipMalloc(&h->d_off1[spin], sizeof(long long) * (n + 1)));
+        JCI_CHECK(hipMalloc(&h->d_off2[spin], sizeof(long long) * (n + 1)));
+        RCHECK(jk_exc_count(s->d_desc, spin, n, s->d_mask[spin], c->d_binom, c1, c2, c->stream));
+        RCHECK(jk_scan(c1, h->d_off1[spin], n, c->stream));
+        RCHECK(jk_scan(c2, h->d_off2[spin], n, c->stream));
+        JCI_CHECK(hipMemcpyAsync(&h->nexc1[spin], h->d_off1[spin] + n, sizeof(long long),
+                                 hipMemcpyDeviceToHost, c->stream));
+        JCI_CHECK(hipMemcpyAsync(&h->nexc2[spin], h->d_off2[spin] + n, sizeof(long long),
+                                 hipMemcpyDeviceToHost, c->stream));
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+        hipFree(c1);
+        hipFree(c2);
+        size_t a1 = h->nexc1[spin] > 0 ? h->nexc1[spin] : 1, a2 = h->nexc2[spin] > 0 ? h->nexc2[spin] : 1;
+        if (hipMalloc(&h->d_e1[spin], sizeof(int2) * a1) != hipSuccess ||
+            hipMalloc(&h->d_v1[spin], sizeof(double) * a1) != hipSuccess ||
+            hipMalloc(&h->d_e2[spin], sizeof(int2) * a2) != hipSuccess ||
+            hipMalloc(&h->d_v2[spin], sizeof(double) * a2) != hipSuccess) {
+            jci_set_error("excitation tables: out of device memory (%lld + %lld entries)",
+                          h->nexc1[spin], h->nexc2[spin]);
+            return -3;
+        }
+        RCHECK(jk_exc_fill(s->d_desc, spin, n, s->d_mask[spin], c->d_binom, h->d_m1, h->d_m2,
+                           h->d_off1[spin], h->d_off2[spin], h->d_e1[spin], h->d_v1[spin],
+                           h->d_e2[spin], h->d_v2[spin], c->stream));
+    }
+    if (same) {
+        h->d_off1[1] = h->d_off1[0]; h->d_off2[1] = h->d_off2[0];
+        h->d_e1[1] = h->d_e1[0]; h->d_e2[1] = h->d_e2[0];
+        h->d_v1[1] = h->d_v1[0]; h->d_v2[1] = h->d_v2[0];
+        h->nexc1[1] = h->nexc1[0]; h->nexc2[1] = h->nexc2[0];
+    }
+    int n = D.norbs, mp = n * (n + 1) / 2;
+    JCI_CHECK(hipMalloc(&h->d_m2sq, sizeof(double) * (size_t)mp * mp));
+    RCHECK(jk_m2_square(mp, h->d_m2, h->d_m2sq, c->stream));
+    const int widths[3] = {64, 128, 256};
+    for (int wi = 0; wi < 3; wi++) {
+        int Wd = widths[wi];
+        std::vector<long long> toff(D.nblk + 1, 0);
+        for (int b = 0; b < D.nblk; b++) {
+            long long na = D.s[0].gn[D.bp[b]], nq = D.s[1].gn[D.bq[b]];
+            toff[b + 1] = toff[b] + na * ((nq + Wd - 1) / Wd);
+        }
+        JCI_CHECK(hipMalloc(&h->d_tile_off[wi], sizeof(long long) * (D.nblk + 1)));
+        JCI_CHECK(hipMemcpyAsync(h->d_tile_off[wi], toff.data(), sizeof(long long) * (D.nblk + 1),
+                                 hipMemcpyHostToDevice, c->stream));
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+        FastArgs &a = h->fast[wi];
+        a.d = s->d_desc;
+        a.ma = s->d_mask[0];
+        a.mb = s->d_mask[1];
+        a.ta = ExcView{h->d_off1[0], h->d_off2[0], h->d_e1[0], h->d_e2[0], h->d_v1[0], h->d_v2[0]};
+        a.tb = ExcView{h->d_off1[1], h->d_off2[1], h->d_e1[1], h->d_e2[1], h->d_v1[1], h->d_v2[1]};
+        a.m2sq = h->d_m2sq;
+        a.hd = h->d_hd;
+        a.tile_off = h->d_tile_off[wi];
+        a.m1 = mp;
+        a.W = Wd;
+        a.ntiles = toff[D.nblk];
+        if (Wd == W) h->fast_w = wi;
+    }
+    h->have_fast = 1;
+    return 0;
+}
+
 JCI_API int jci_ham_create(jci_space *s, const double *m1, const double *m2, int flags,
                               jci_ham **out)
 {
@@ -313,6 +401,10 @@ JCI_API int jci_ham_create(jci_space *s, const double *m1, const double *m2, int
         int rc = build_pairs(h, flags);
         if (rc) { jci_ham_destroy(h); return rc; }
     }
+    if (flags & JCI_DIRECT_TABLES) {
+        int rc = build_fast(h, 256);
+        if (rc) { jci_ham_destroy(h); return rc; }
+    }
     JCI_CHECK(hipStreamSynchronize(c->stream));
     *out = h;
     return 0;
@@ -326,6 +418,13 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     hipFree(h->d_m1); hipFree(h->d_m2); hipFree(h->d_hd);
     hipFree(h->d_rowptr); hipFree(h->d_col); hipFree(h->d_val64); hipFree(h->d_val32);
     hipFree(h->d_xf);
+    for (int sp = 0; sp < 2; sp++) {
+        if (sp == 1 && h->d_off1[1] == h->d_off1[0]) break;
+        hipFree(h->d_off1[sp]); hipFree(h->d_off2[sp]); hipFree(h->d_e1[sp]);
+        hipFree(h->d_e2[sp]); hipFree(h->d_v1[sp]); hipFree(h->d_v2[sp]);
+    }
+    hipFree(h->d_m2sq);
+    for (int wi = 0; wi < 3; wi++) hipFree(h->d_tile_off[wi]);
     jci_space *s = h->sp;
     free(h);
     if (--s->refs == 0 && s->closing) space_free(s);
@@ -378,9 +477,14 @@ JCI_API int jci_ham_rows(jci_ham *h, const long long *rows, int nrows, long long
 static int resolve_method(const jci_ham *h, int method, int want32)
 {
     if (method != JCI_SIGMA_AUTO) return method;
-    if (want32) return h->d_val32 ? JCI_SIGMA_PAIRS32 : JCI_SIGMA_DIRECT32;
+    if (want32 && h->tuned32) return h->tuned32;
+    if (!want32 && h->tuned64) return h->tuned64;
+    if (want32) {
+        if (h->d_val32) return JCI_SIGMA_PAIRS32;
+        return h->have_fast ? JCI_SIGMA_FAST32 : JCI_SIGMA_DIRECT32;
+    }
     if (h->d_val64) return h->upper ? JCI_SIGMA_PAIRS_SYM64 : JCI_SIGMA_PAIRS64;
-    return JCI_SIGMA_DIRECT64;
+    return h->have_fast ? JCI_SIGMA_FAST64 : JCI_SIGMA_DIRECT64;
 }
 
 static int ensure_xf(jci_ham *h, long long n)
@@ -428,6 +532,14 @@ static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *
         RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
         return jk_sigma_generic(32, nvec, s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom,
                                 h->d_m1, h->d_m2, h->d_hd, h->d_xf, y, c->stream);
+    case JCI_SIGMA_FAST64:
+        if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
+        return jk_sigma_fast(64, nvec, h->fast[h->fast_w], x, y, c->stream);
+    case JCI_SIGMA_FAST32:
+        if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
+        RCHECK(ensure_xf(h, N * nvec));
+        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
+        return jk_sigma_fast(32, nvec, h->fast[h->fast_w], h->d_xf, y, c->stream);
     default:
         jci_set_error("sigma: unknown method %d", method);
         return -1;
@@ -565,6 +677,51 @@ JCI_API int jci_bench_sigma(jci_ham *h, int method, int nvec, int reps, double *
     hipFree(x);
     hipFree(y);
     return rc;
+}
+
+// Times every available sigma variant (pair list, matrix-free with tile widths 64/128/256,
+// one-thread-per-row) for nvec vectors and makes the fastest FP64 and FP32 ones the
+// defaults of JCI_SIGMA_AUTO.  times[] (may be NULL) receives ms per launch indexed by
+// JCI_SIGMA_* for the chosen tile width (0 where not available).
+JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32,
+                             double *times)
+{
+    double t[8] = {0};
+    double best_w_ms = 1e300;
+    int best_w = h->fast_w;
+    if (h->have_fast) {
+        for (int wi = 0; wi < 3; wi++) {
+            h->fast_w = wi;
+            double ms = 0;
+            RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST64, nvec, reps, &ms));
+            if (ms < best_w_ms) { best_w_ms = ms; best_w = wi; }
+        }
+        h->fast_w = best_w;
+        t[JCI_SIGMA_FAST64] = best_w_ms;
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST32, nvec, reps, &t[JCI_SIGMA_FAST32]));
+    }
+    if (h->d_val64 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS64, nvec, reps, &t[JCI_SIGMA_PAIRS64]));
+    if (h->d_val32 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS32, nvec, reps, &t[JCI_SIGMA_PAIRS32]));
+    if (!h->have_fast && !h->d_val64 && !h->d_val32) {
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_DIRECT64, nvec, 1, &t[JCI_SIGMA_DIRECT64]));
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_DIRECT32, nvec, 1, &t[JCI_SIGMA_DIRECT32]));
+    }
+    int b64 = 0, b32 = 0;
+    const int c64[] = {JCI_SIGMA_PAIRS64, JCI_SIGMA_FAST64, JCI_SIGMA_DIRECT64};
+    const int c32[] = {JCI_SIGMA_PAIRS32, JCI_SIGMA_FAST32, JCI_SIGMA_DIRECT32};
+    for (int m : c64) if (t[m] > 0 && (b64 == 0 || t[m] < t[b64])) b64 = m;
+    for (int m : c32) if (t[m] > 0 && (b32 == 0 || t[m] < t[b32])) b32 = m;
+    h->tuned64 = b64;
+    h->tuned32 = b32;
+    if (best64) *best64 = b64;
+    if (best32) *best32 = b32;
+    if (times) for (int m = 0; m < 8; m++) times[m] = t[m];
+    return 0;
+}
+
+JCI_API int jci_ham_fast_width(const jci_ham *h)
+{
+    return h->have_fast ? h->fast[h->fast_w].W : 0;
 }
 
 JCI_API int jci_bench_copy(jci_ctx *c, long long bytes, int reps, double *ms)

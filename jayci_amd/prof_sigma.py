@@ -5,8 +5,9 @@
 
 Order of launches (each preceded by one warm-up launch): a 1 GiB device copy (k_copy,
 the byte-counter calibration: 1 GiB read + 1 GiB written per launch), the FP64 pair-list
-sigma (k_spmv<double>), the FP32 pair-list sigma (k_spmv<float>), the symmetric
-upper-triangle sigma (k_spmv_sym) and the matrix-free sigma (k_sigma_generic<double>).
+sigma (k_spmv<double>), the FP32 pair-list sigma (k_spmv<float>), the matrix-free
+table sigma (k_sigma_fast, autotuned over tile widths), the symmetric upper-triangle sigma
+(k_spmv_sym) and the one-thread-per-row matrix-free sigma (k_sigma_generic<double>).
 """
 import argparse
 import os
@@ -30,10 +31,16 @@ def main():
     m1, m2 = J.synth_integrals(wl["norbs"], wl["seed"])
     sp = J.Space(ctx, wl["na"], wl["nb"], wl["norbs"], wl["ndocc"], wl["nactv"], wl["xlvl"], wl["rule"])
     print("copy 1GiB  ms %.4f" % ctx.bench_copy(1 << 30, a.reps), flush=True)
-    h = J.Hamiltonian(sp, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32)
+    h = J.Hamiltonian(sp, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32 | J.DIRECT_TABLES)
     print("N %d pairs %d" % (sp.ndets, h.npairs), flush=True)
     print("spmv64     ms %.4f" % h.bench_sigma(J.SIGMA_PAIRS64, 1, a.reps), flush=True)
     print("spmv32     ms %.4f" % h.bench_sigma(J.SIGMA_PAIRS32, 1, a.reps), flush=True)
+    b64, b32, t, w = h.autotune(1, a.reps)
+    print("autotune   %s best64 %d best32 %d W %d" % (t, b64, b32, w), flush=True)
+    print("fast64     ms %.4f" % h.bench_sigma(J.SIGMA_FAST64, 1, a.reps), flush=True)
+    print("fast32     ms %.4f" % h.bench_sigma(J.SIGMA_FAST32, 1, a.reps), flush=True)
+    print("fast64 k=3 ms %.4f" % h.bench_sigma(J.SIGMA_FAST64, 3, a.reps), flush=True)
+    print("spmv64 k=3 ms %.4f" % h.bench_sigma(J.SIGMA_PAIRS64, 3, a.reps), flush=True)
     h.close()
     hu = J.Hamiltonian(sp, m1, m2, J.PAIRS_FP64 | J.PAIRS_UPPER)
     print("spmv_sym   ms %.4f" % hu.bench_sigma(J.SIGMA_PAIRS_SYM64, 1, a.reps), flush=True)

@@ -59,14 +59,14 @@ def test_rows_bitexact(J, orc, ctx, spec):
         np.testing.assert_allclose(v, ov, rtol=1e-12, atol=1e-13)
 
 
-METHODS64 = ["SIGMA_PAIRS64", "SIGMA_PAIRS_SYM64", "SIGMA_DIRECT64"]
-METHODS32 = ["SIGMA_PAIRS32", "SIGMA_DIRECT32"]
+METHODS64 = ["SIGMA_PAIRS64", "SIGMA_PAIRS_SYM64", "SIGMA_DIRECT64", "SIGMA_FAST64"]
+METHODS32 = ["SIGMA_PAIRS32", "SIGMA_DIRECT32", "SIGMA_FAST32"]
 
 
 @pytest.mark.parametrize("spec", SMALL_SPACES)
 def test_sigma_all_methods(J, orc, ctx, spec):
     gs, os_, m1, m2 = _spaces(J, orc, ctx, spec)
-    full = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32)
+    full = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32 | J.DIRECT_TABLES)
     upper = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64 | J.PAIRS_UPPER)
     N = gs.ndets
     X = rand_vectors(N, 3, 5)
@@ -116,12 +116,13 @@ def test_davidson_vs_dense(J, orc, ctx, spec):
 @pytest.mark.parametrize("flags,sig64,sig32", [
     ("pairs", "SIGMA_PAIRS64", "SIGMA_PAIRS32"),
     ("none", "SIGMA_DIRECT64", "SIGMA_DIRECT32"),
+    ("tables", "SIGMA_FAST64", "SIGMA_FAST32"),
 ])
 def test_davidson_precision_and_methods(J, orc, ctx, precision, flags, sig64, sig32):
     gs, os_, m1, m2 = _spaces(J, orc, ctx, (3, 3, 8, 2, 2, 2, 1), seed=3)
     H = os_.dense_h(m1, m2)
     w = np.linalg.eigh(H)[0]
-    fl = J.PAIRS_FP64 | J.PAIRS_FP32 if flags == "pairs" else 0
+    fl = {"pairs": J.PAIRS_FP64 | J.PAIRS_FP32, "none": 0, "tables": J.DIRECT_TABLES}[flags]
     h = J.Hamiltonian(gs, m1, m2, fl)
     ev, vec, st = h.solve(nroots=3, restol=1e-7, krymin=4, krymax=12, refdim=40,
                           precision=getattr(J, precision), sigma_fp64=getattr(J, sig64),
