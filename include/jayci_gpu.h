@@ -248,6 +248,27 @@ int jci_ham_fast_width(const jci_ham *h);
  * eigenvalues ascending, eigenvector i in vec[i*n + 0..n-1] (largest component positive) */
 int jci_jacobi(int n, const double *a, double *eval, double *vec);
 
+/* ---- batched solves (many independent CI instances in lockstep) ---------------------------
+ * Every active slot advances one Davidson iteration per jci_batch_step; the sigma of all
+ * slots' new vectors is one persistent launch (recorded as a HIP graph per precision and
+ * block width and replayed).  Slots are filled between steps (continuous batching) and
+ * report done when converged.  Hamiltonians need JCI_DIRECT_TABLES.  max_norbs bounds the
+ * orbitals of any instance (LDS sizing). */
+typedef struct jci_batch jci_batch;
+int jci_batch_create(jci_ctx *ctx, int capacity, int max_norbs, jci_batch **out);
+void jci_batch_destroy(jci_batch *b);
+int jci_batch_set_graphs(jci_batch *b, int on);
+/* device bytes a slot of ndets determinants takes (basis, sigma and staging vectors) */
+long long jci_batch_bytes(long long ndets, int krymax);
+int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, int *slot);
+int jci_batch_step(jci_batch *b, int *nactive);
+int jci_batch_run(jci_batch *b, int max_steps);
+/* done: 0 running, 1 converged, 2 stopped (maxiter or no new directions) */
+int jci_batch_result(jci_batch *b, int slot, int *done, double *evals, double *evecs,
+                     jci_solve_stats *stats);
+int jci_batch_release(jci_batch *b, int slot);
+double jci_batch_sigma_seconds(const jci_batch *b);
+
 /* Synthetic integrals: h_ii = -2.0 + 0.25 i (1-based i), h_ij ~ N(0, 0.05^2);
  * (ij|kl) = sum_{P=1..2n} L^P_ij L^P_kl with L^P symmetric, L^P_ii ~ U(0.1, 0.3),
  * L^P_ij ~ N(0, 0.02^2).  SplitMix64 stream from `seed`, Box-Muller normals.

@@ -84,6 +84,16 @@ _sig("jci_bench_sigma", _i, [_P, _i, _i, _i, ctypes.POINTER(_d)])
 _sig("jci_bench_copy", _i, [_P, _ll, _i, ctypes.POINTER(_d)])
 _sig("jci_ham_autotune", _i, [_P, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _P])
 _sig("jci_ham_fast_width", _i, [_P])
+_sig("jci_batch_create", _i, [_P, _i, _i, ctypes.POINTER(_P)])
+_sig("jci_batch_destroy", None, [_P])
+_sig("jci_batch_set_graphs", _i, [_P, _i])
+_sig("jci_batch_bytes", _ll, [_ll, _i])
+_sig("jci_batch_add", _i, [_P, _P, ctypes.POINTER(SolveParams), ctypes.POINTER(_i)])
+_sig("jci_batch_step", _i, [_P, ctypes.POINTER(_i)])
+_sig("jci_batch_run", _i, [_P, _i])
+_sig("jci_batch_result", _i, [_P, _i, ctypes.POINTER(_i), _P, _P, ctypes.POINTER(SolveStats)])
+_sig("jci_batch_release", _i, [_P, _i])
+_sig("jci_batch_sigma_seconds", _d, [_P])
 
 
 class JciError(RuntimeError):
@@ -335,6 +345,12 @@ class Hamiltonian:
         Y = np.ascontiguousarray(Yi.T)
         return Y[0] if single else Y
 
+    def params(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
+               precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
+               sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False):
+        return make_params(self.space.ndets, nroots, restol, maxiter, krymin, krymax, refdim,
+                           precision, mixed_switch, guess, totfrze, sigma_fp64, sigma_fp32, verbose)
+
     def solve(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
               precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, guess_vectors=None,
               totfrze=0.0, sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, vectors=True,
@@ -369,3 +385,86 @@ class Hamiltonian:
                  "rnorm": [st.rnorm[i] for i in range(nroots)], "seconds": st.seconds,
                  "sigma_seconds": st.sigma_seconds}
         return ev, vec, stats
+
+
+def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
+                precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
+                sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False):
+    p = SolveParams()
+    _L.jci_solve_defaults(ctypes.byref(p))
+    p.nroots = nroots
+    p.restol = restol
+    p.maxiter = maxiter
+    p.krymin = krymin if krymin is not None else max(nroots + 1, min(4, ndets))
+    p.krymax = krymax if krymax is not None else min(max(p.krymin + 2 * nroots, 16), 64)
+    p.refdim = refdim
+    p.precision = precision
+    p.mixed_switch = mixed_switch
+    p.guess = guess
+    p.totfrze = totfrze
+    p.sigma_fp64 = sigma_fp64
+    p.sigma_fp32 = sigma_fp32
+    p.verbose = 1 if verbose else 0
+    return p
+
+
+class Batch:
+    """Lockstep batched Davidson over many instances (continuous batching)."""
+
+    def __init__(self, ctx, capacity, max_norbs=64, graphs=True):
+        self.ctx = ctx
+        h = _P()
+        _check(_L.jci_batch_create(ctx._h, capacity, max_norbs, ctypes.byref(h)), "jci_batch_create")
+        self._h = h
+        self._keep = {}
+        if not graphs:
+            _L.jci_batch_set_graphs(h, 0)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _L.jci_batch_destroy(self._h)
+            self._h = None
+            self._keep = {}
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def bytes_for(ndets, krymax):
+        return _L.jci_batch_bytes(ndets, krymax)
+
+    def add(self, ham, params):
+        slot = _i()
+        _check(_L.jci_batch_add(self._h, ham._h, ctypes.byref(params), ctypes.byref(slot)),
+               "jci_batch_add")
+        self._keep[slot.value] = (ham, params)
+        return slot.value
+
+    def step(self):
+        n = _i()
+        _check(_L.jci_batch_step(self._h, ctypes.byref(n)), "jci_batch_step")
+        return n.value
+
+    def run(self, max_steps=1000):
+        _check(_L.jci_batch_run(self._h, max_steps), "jci_batch_run")
+
+    def result(self, slot, vectors=False):
+        ham, params = self._keep[slot]
+        done = _i()
+        ev = np.zeros(params.nroots)
+        vec = np.zeros((params.nroots, ham.space.ndets)) if vectors else None
+        st = SolveStats()
+        _check(_L.jci_batch_result(self._h, slot, ctypes.byref(done), _p(ev), _p(vec),
+                                   ctypes.byref(st)), "jci_batch_result")
+        stats = {"done": done.value, "iterations": st.iterations, "converged": bool(st.converged),
+                 "nsigma64": st.nsigma64, "nsigma32": st.nsigma32,
+                 "rnorm": [st.rnorm[i] for i in range(params.nroots)], "seconds": st.seconds}
+        return ev, vec, stats
+
+    def release(self, slot):
+        _check(_L.jci_batch_release(self._h, slot), "jci_batch_release")
+        self._keep.pop(slot, None)
+
+    @property
+    def sigma_seconds(self):
+        return _L.jci_batch_sigma_seconds(self._h)

@@ -753,6 +753,11 @@ JCI_API int jci_bench_copy(jci_ctx *c, long long bytes, int reps, double *ms)
     return rc;
 }
 
+void jci_jacobi_eig(int n, const double *a, double *eval, double *vec)
+{
+    jacobi_eig(n, a, eval, vec);
+}
+
 JCI_API int jci_jacobi(int n, const double *a, double *eval, double *vec)
 {
     if (n < 1) return -1;
@@ -1100,6 +1105,43 @@ JCI_API int jci_solve(jci_ham *h, const jci_solve_params *p, const double *guess
     op.masks = ham_masks;
     op.resolve = ham_resolve;
     return jci_solve_core(&op, p, guess, evals, evecs, stats);
+}
+
+// ---- accessors for the batched engine ---------------------------------------------------------
+int jci_ham_fastargs(jci_ham *h, FastArgs *out)
+{
+    if (!h->have_fast) { jci_set_error("batch: the Hamiltonian has no excitation tables"); return -1; }
+    *out = h->fast[2];            // the batched sigma uses 256-string beta tiles
+    return 0;
+}
+const double *jci_ham_hd(jci_ham *h) { return h->d_hd; }
+long long jci_ham_ndets(jci_ham *h) { return h->sp->h.ndets; }
+jci_ctx *jci_ham_ctx(jci_ham *h) { return h->sp->ctx; }
+
+// initial guess block of a Hamiltonian into V (nguess vectors, leading dimension ld)
+int jci_ham_guess_block(jci_ham *h, int refdim, int nguess, int diag_guess, double *V, long long ld)
+{
+    SolveOp op;
+    op.self = h;
+    op.ctx = h->sp->ctx;
+    op.N = h->sp->h.ndets;
+    op.d_hd = h->d_hd;
+    op.d_m1 = h->d_m1;
+    op.d_m2 = h->d_m2;
+    op.sigma = ham_sigma;
+    op.masks = ham_masks;
+    op.resolve = ham_resolve;
+    Work w{};
+    w.op = &op;
+    w.c = op.ctx;
+    w.N = op.N;
+    w.ld = ld;
+    w.V = V;
+    jci_solve_params p;
+    jci_solve_defaults(&p);
+    p.refdim = refdim;
+    p.guess = diag_guess ? JCI_GUESS_DIAG : JCI_GUESS_BLOCK;
+    return guess_block(w, &p, nguess);
 }
 
 // ---- synthetic integrals ------------------------------------------------------------------

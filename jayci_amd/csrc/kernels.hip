@@ -633,15 +633,26 @@ __global__ __launch_bounds__(256) void k_gram(int m, int w, long long N,
     }
 }
 
+// one wavefront per output element: lanes take the partials b = lane, lane + 64, ... and a
+// butterfly finishes the sum (fixed order, deterministic)
 __global__ void k_gram_reduce(int m, int w, int nblocks, const double *__restrict__ partial,
                               double *__restrict__ out)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int idx = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (idx >= m * w) return;
     int i = idx / w, j = idx % w;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partial[(long long)b * JCI_MAXBASIS * JCI_MAXBASIS + i * JCI_MAXBASIS + j];
-    out[i * w + j] = s;
+    for (int b = lane; b < nblocks; b += 64)
+        s += partial[(long long)b * JCI_MAXBASIS * JCI_MAXBASIS + i * JCI_MAXBASIS + j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) out[i * w + j] = s;
+}
+
+static int reduction_blocks(long long N, int cap)
+{
+    long long b = (N + 8191) / 8192;
+    if (b > cap) b = cap;
+    return (int)(b < 1 ? 1 : b);
 }
 
 int jk_gram(int m, int w, long long N, const double *V, long long ldv, const double *W,
@@ -655,12 +666,13 @@ int jk_gram(int m, int w, long long N, const double *V, long long ldv, const dou
         jci_set_error("gram: at most 16 output tiles");
         return -1;
     }
+    nblocks = reduction_blocks(N, nblocks);
     long long chunk = (N + nblocks - 1) / nblocks;
     chunk = (chunk + GRAM_ROWS - 1) / GRAM_ROWS * GRAM_ROWS;
     int nb = (int)((N + chunk - 1) / chunk);
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_gram, dim3(nb), dim3(256), 0, st, m, w, N, V, ldv, W, ldw, partial, chunk);
-    hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(m * w, 256)), dim3(256), 0, st, m, w, nb,
+    hipLaunchKernelGGL(k_gram_reduce, dim3(grid_for(m * w, 4)), dim3(256), 0, st, m, w, nb,
                        partial, out_dev);
     LAUNCH_CHECK();
     return 0;
@@ -745,14 +757,16 @@ __global__ __launch_bounds__(256) void k_residual(int m, int k, long long N,
     if (threadIdx.x < k) partial[(long long)blockIdx.x * JCI_MAXK + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// out[r] = sum_b partial[b*JCI_MAXK + r], one wavefront per r (fixed order)
 __global__ void k_partial_reduce(int k, int nblocks, const double *__restrict__ partial,
                                  double *__restrict__ out)
 {
-    int r = threadIdx.x;
+    int r = blockIdx.x, lane = threadIdx.x;
     if (r >= k) return;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partial[(long long)b * JCI_MAXK + r];
-    out[r] = s;
+    for (int b = lane; b < nblocks; b += 64) s += partial[(long long)b * JCI_MAXK + r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) out[r] = s;
 }
 
 int jk_residual(int m, int k, long long N, const double *V, const double *S, long long ld,
@@ -763,11 +777,12 @@ int jk_residual(int m, int k, long long N, const double *V, const double *S, lon
         jci_set_error("residual: dimensions out of range");
         return -1;
     }
+    nblocks = reduction_blocks(N, nblocks);
     long long chunk = (N + nblocks - 1) / nblocks;
     int nb = (int)((N + chunk - 1) / chunk);
     hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, m, k, N, V, S, ld, Y, theta, hd,
                        T, partial, chunk);
-    hipLaunchKernelGGL(k_partial_reduce, dim3(1), dim3(64), 0, st, k, nb, partial, norms_dev);
+    hipLaunchKernelGGL(k_partial_reduce, dim3(k), dim3(64), 0, st, k, nb, partial, norms_dev);
     LAUNCH_CHECK();
     return 0;
 }
@@ -799,10 +814,11 @@ int jk_colnorms(int k, long long N, const double *X, long long ld, double *parti
                 int nblocks, double *out_dev, hipStream_t st)
 {
     if (k < 1 || k > JCI_MAXK) { jci_set_error("colnorms: k out of range"); return -1; }
+    nblocks = reduction_blocks(N, nblocks);
     long long chunk = (N + nblocks - 1) / nblocks;
     int nb = (int)((N + chunk - 1) / chunk);
     hipLaunchKernelGGL(k_colnorms, dim3(nb), dim3(256), 0, st, k, N, X, ld, partial, chunk);
-    hipLaunchKernelGGL(k_partial_reduce, dim3(1), dim3(64), 0, st, k, nb, partial, out_dev);
+    hipLaunchKernelGGL(k_partial_reduce, dim3(k), dim3(64), 0, st, k, nb, partial, out_dev);
     LAUNCH_CHECK();
     return 0;
 }

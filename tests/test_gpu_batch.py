@@ -1,0 +1,81 @@
+"""Batched (lockstep, one sigma launch for all instances) solves against standalone solves
+and dense eigensolutions: every instance's eigenvalues within 1e-8 Ha, alone and in the
+mix, in FP64 and in mixed precision."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (na, nb, norbs, ndocc, nactv, xlvl, rule, nroots)
+INSTANCES = [(2, 2, 6, 0, 6, 0, 1, 1), (3, 3, 8, 0, 8, 0, 1, 2), (3, 2, 7, 0, 7, 0, 1, 3),
+             (3, 3, 8, 2, 2, 2, 1, 2), (4, 4, 9, 0, 9, 0, 1, 4), (3, 3, 7, 1, 3, 2, 0, 1),
+             (4, 3, 9, 1, 4, 1, 0, 2), (2, 2, 5, 0, 5, 0, 1, 2)]
+
+
+def _build(J, orc, ctx, spec, seed):
+    na, nb, n, nd, nact, xl, rule, nr = spec
+    m1, m2 = J.synth_integrals(n, seed)
+    sp = J.Space(ctx, na, nb, n, nd, nact, xl, rule)
+    h = J.Hamiltonian(sp, m1, m2, J.DIRECT_TABLES)
+    w = None
+    if sp.ndets <= 3000:
+        w = np.linalg.eigvalsh(orc.Space(na, nb, n, nd, nact, xl, rule).dense_h(m1, m2))
+    return sp, h, w, nr
+
+
+@pytest.mark.parametrize("precision", ["PREC_FP64", "PREC_MIXED"])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_matches_dense_and_alone(J, orc, ctx, precision, graphs):
+    prec = getattr(J, precision)
+    items = [_build(J, orc, ctx, spec, 100 + i) for i, spec in enumerate(INSTANCES)]
+    b = J.Batch(ctx, 16, 16, graphs=graphs)
+    slots = []
+    for sp, h, w, nr in items:
+        p = h.params(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
+                     refdim=min(48, sp.ndets), precision=prec, guess=J.GUESS_DIAG)
+        slots.append(b.add(h, p))
+    b.run(400)
+    for (sp, h, w, nr), s in zip(items, slots):
+        ev, vec, st = b.result(s, vectors=True)
+        assert st["done"] == 1, st
+        alone, _, _ = h.solve(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
+                              refdim=min(48, sp.ndets), precision=prec, guess=J.GUESS_DIAG)
+        np.testing.assert_allclose(ev, alone, rtol=0, atol=1e-8)
+        if w is not None:
+            np.testing.assert_allclose(ev, w[:nr], rtol=0, atol=1e-8)
+        hv = h.sigma(vec, method=J.SIGMA_FAST64)
+        for r in range(nr):
+            assert np.linalg.norm(hv[r] - ev[r] * vec[r]) < 2e-7
+        if precision == "PREC_MIXED":
+            assert st["nsigma32"] > 0
+        b.release(s)
+    b.close()
+
+
+def test_batch_continuous_admission(J, orc, ctx):
+    """Instances added while others are mid-solve give the same result as alone."""
+    items = [_build(J, orc, ctx, spec, 200 + i) for i, spec in enumerate(INSTANCES)]
+    b = J.Batch(ctx, 4, 16)
+    pending = list(range(len(items)))
+    active = {}
+    results = {}
+    steps = 0
+    while pending or active:
+        while pending and len(active) < 4:
+            i = pending.pop(0)
+            sp, h, w, nr = items[i]
+            p = h.params(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
+                         refdim=min(48, sp.ndets), precision=J.PREC_MIXED, guess=J.GUESS_DIAG)
+            active[b.add(h, p)] = i
+        b.step()
+        steps += 1
+        for s in list(active):
+            ev, _, st = b.result(s)
+            if st["done"]:
+                results[active.pop(s)] = ev
+                b.release(s)
+        assert steps < 2000
+    for i, (sp, h, w, nr) in enumerate(items):
+        alone, _, _ = h.solve(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
+                              refdim=min(48, sp.ndets), precision=J.PREC_MIXED, guess=J.GUESS_DIAG)
+        np.testing.assert_allclose(results[i], alone, rtol=0, atol=1e-8)
