@@ -1,21 +1,31 @@
-"""Benchmark: Davidson solves/s and latency, mixed precision vs plain FP64, one GPU.
+"""Benchmark: Davidson solves/s and p99 latency at 1e-8 Ha, mixed vs plain FP64, one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fci12]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload served|batched|fci12]
 
-A step is one complete Davidson solve (lowest root(s) to residual 1e-6 and 1e-8 Ha of the
-golden energy) on synthetic integrals resident on the GPU.  K solves of the mixed
-(FP32 sigma -> FP64 refinement) path are timed, then K solves of the plain FP64 path on
-the same inputs for the comparison; value = mixed solves/s.
+Workloads (synthetic integrals from seeds, resident on the GPU before each solve starts):
+  served   (default) a seeded Poisson stream (mean 200 requests/s, seed 400) of K CI
+           solves drawn like the batched configuration: full CI, orbitals n ~ U{8..14},
+           alpha = beta electrons ~ U{3..5}, roots ~ U{1..4}, residual 1e-6.  Timed on the
+           same stream for
+           (a) the served path: continuous batching (one sigma launch per lockstep
+               iteration for every resident request), mixed precision (FP32 sigma ->
+               FP64 refinement), admission by slots and a 200 GB HBM budget  -> value;
+           (b) the plain FP64 GPU path: one request at a time in arrival order;
+           (c) the served path in FP64 (isolates batching from precision).
+           A step is one request; ms_per_step = wall time of (a) / K.
+  batched  the same distribution, all K instances present at t = 0 (seed 300).
+  fci12    full CI 4a4b / 12 orbitals (245,025 determinants), 1 root: K solves per path.
 
-The line also carries the roofline of the dominant kernel (the pair-list sigma, timed
-with HIP events on its own stream inside the library) and the CPU baseline (the
-reference's own compute_hv, compiled from its sources into oracle/_ref, timed on this
-host for one sigma pass and scaled by the sigma passes of a plain solve).
+Each root of every path is compared with a tight FP64 GPU solve of the same instance
+(residual 1e-9, FP64 sigma); max |dE| is reported and the target is <= 1e-8 Ha.  The
+roofline is that of the plain sigma kernel (the FP64 pair list, k_spmv<double>) on the
+full-CI 4a4b/12 space, timed with HIP events on the library's stream.  The CPU baseline
+is the reference's compute_hv (oracle/_ref) timed for one sigma pass over a sample
+instance and converted to solves/s through the workload's total sigma work.
 """
 import argparse
 import json
 import os
-import statistics
 import sys
 import time
 
@@ -23,11 +33,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12        # MI355X HBM3E, bytes/s
-
-from jayci_amd.workloads import WORKLOADS  # noqa: E402  (plain dict module)
-
+METRIC = "Davidson solves/s + p99 latency at 1e-8 Ha, mixed vs plain FP64, 1 GPU"
 METHOD_NAMES = {1: "pairs64", 2: "pairs32", 3: "pairs_sym64", 4: "direct64", 5: "direct32",
-                6: "fast64", 7: "fast32"}
+                6: "fast64", 7: "fast32", 8: "fci64", 9: "fci32"}
+
+
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def percentile(xs, q):
@@ -40,32 +52,52 @@ def percentile(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
-def golden_energy(name):
-    path = os.path.join(ROOT, "tests", "golden", "cfg0_energies.json")
-    if name == "fci12" and os.path.exists(path):
-        return json.load(open(path))["energies"]
-    return None
+def solve_kw(J, spec, N, restol):
+    nr = spec["nroots"]
+    return dict(nroots=nr, restol=restol, krymin=min(nr + 1, N),
+                krymax=min(max(3 * nr + 1, 12), N), refdim=min(64, N), guess=J.GUESS_DIAG,
+                totfrze=J.SYNTH_ENUC, vectors=False)
 
 
-def cpu_baseline(wl, m1, m2, nsigma_per_solve):
-    """The reference's serial compute_hv (source/action_util.c:121, OpenMP) for one sigma
-    pass over the whole space, scaled to solves/s by the sigma passes of a plain GPU
-    solve.  Its connectivity map comes from this library's generate_cimap (same section
-    lists as source/cimapping.c:92; checked bit-exact in the tests)."""
+def make_instance(J, ctx, spec, seed, flags):
+    m1, m2 = J.synth_integrals(spec["norbs"], seed)
+    sp = J.Space(ctx, spec["na"], spec["nb"], spec["norbs"], spec["ndocc"], spec["nactv"],
+                 spec["xlvl"], spec["rule"])
+    return sp, J.Hamiltonian(sp, m1, m2, flags)
+
+
+def fci_pairs(spec):
+    """(N, unique off-diagonal pairs) of a full-CI space: c = 2a(n-a) + 2C(a,2)C(n-a,2)
+    + (a(n-a))^2 connections per row for a = alpha = beta electrons."""
+    from math import comb
+    n, a, b = spec["norbs"], spec["na"], spec["nb"]
+    N = comb(n, a) * comb(n, b)
+    c = (a * (n - a) + b * (n - b) + comb(a, 2) * comb(n - a, 2) + comb(b, 2) * comb(n - b, 2)
+         + a * (n - a) * b * (n - b))
+    return N, N * c // 2
+
+
+def cpu_baseline(J, sample, seed, work):
+    """The reference's compute_hv (source/action_util.c:121, OpenMP) timed for one sigma pass
+    over one sample instance; its rate in Hamiltonian elements per second (P_u + N per pass)
+    converts the workload's total sigma work into solves/s.  work = (solves, elements),
+    elements = sum over solves of (P_u + N) x sigma applications of the plain FP64 GPU solve.
+    The connectivity map comes from this library's generate_cimap (the section lists of
+    source/cimapping.c:92, checked bit-exact in the tests)."""
     import ctypes
     import numpy as np
     from oracle import ref
     if not ref.available():
         return None
     os.environ.setdefault("OMP_SCHEDULE", "dynamic")
-    import jayci_amd as J
-    rs = ref.RefSpace(wl["na"], wl["nb"], wl["norbs"], wl["ndocc"], wl["nactv"], wl["xlvl"])
+    m1, m2 = J.synth_integrals(sample["norbs"], seed)
+    rs = ref.RefSpace(sample["na"], sample["nb"], sample["norbs"], sample["ndocc"],
+                      sample["nactv"], sample["xlvl"])
     hmap = ref._libc.malloc(ctypes.sizeof(ref.RowMap) * rs.ndets)
     gen = J.lib().generate_cimap
     gen.restype = ctypes.c_int
     gen.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
-    rc = gen(ctypes.cast(rs.dlist, ctypes.c_void_p), rs.ndets, rs.ninto, hmap)
-    if rc != 0:
+    if gen(ctypes.cast(rs.dlist, ctypes.c_void_p), rs.ndets, rs.ninto, hmap) != 0:
         return None
     v = np.random.default_rng(0).standard_normal(rs.ndets)
     t0 = time.time()
@@ -74,130 +106,244 @@ def cpu_baseline(wl, m1, m2, nsigma_per_solve):
     dealloc = J.lib().deallocate_cimap
     dealloc.argtypes = [ctypes.c_void_p, ctypes.c_int]
     dealloc(hmap, rs.ndets)
+    N, P_u = fci_pairs(sample)
+    rate = (P_u + N) / t
+    solves, elements = work
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / (t * nsigma_per_solve), "unit": "solves/s", "cores": cores,
+    return {"value": solves * rate / elements, "unit": "solves/s", "cores": cores,
             "kind": "reference",
-            "sample": "one full compute_hv pass of the reference (oracle/_ref, gcc -O2 "
-                      "-fopenmp) over the %d-determinant space: %.2f s; scaled by the %d "
-                      "sigma applications of a plain FP64 GPU solve" % (rs.ndets, t,
-                                                                        nsigma_per_solve)}
+            "sample": "one compute_hv pass of the reference (oracle/_ref, gcc -O2 -fopenmp, "
+                      "OMP_SCHEDULE=dynamic) over a %d-determinant full-CI instance "
+                      "(%da%db/%d): %.2f s = %.3g matrix elements/s; the workload's %d solves "
+                      "need %.4g elements (P_u + N per sigma application, sigma counts of "
+                      "the plain FP64 GPU solves)" % (N, sample["na"], sample["nb"],
+                                                        sample["norbs"], t, rate, solves,
+                                                        elements)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="fci12", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
-              file=sys.stderr)
-        sys.exit(2)
+def roofline_of(J, ctx, spec, seed):
+    """FP64 pair-list sigma (k_spmv<double>, one vector) on one instance: algorithmic bytes
+    B = P_u*(8+4) + N*2*8 + 8*N over the HIP-event time of one launch."""
+    sp, h = make_instance(J, ctx, spec, seed, J.PAIRS_FP64 | J.DIRECT_TABLES)
+    N = sp.ndets
+    P_u = h.npairs // 2
+    B = P_u * 12 + N * 16 + 8 * N
+    ms = h.bench_sigma(J.SIGMA_PAIRS64, 1, 20)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sigma_pairs64_fci12_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("workload") == "fci12" and N == 245025:
+            traffic = t.get("bytes_per_launch")
+    ach = B / (ms * 1e-3)
+    out = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+           "frac": ach / PEAK_HBM, "traffic": traffic,
+           "kernel": "k_spmv<double> (FP64 pair list, 1 vector), N=%d (%da%db/%d)"
+                     % (N, spec["na"], spec["nb"], spec["norbs"]),
+           "ms_per_launch": ms, "algorithmic_bytes": B, "pairs_unique": P_u}
+    h.close()
+    sp.close()
+    return out
 
-    import numpy as np
-    import jayci_amd as J
 
-    wl = WORKLOADS[args.workload]
-    ctx = J.Context(0)
+def run_plain(J, ctx, stream, restol, precision):
+    """One request at a time in arrival order (no batching).  Returns (results, wall)."""
+    out = []
+    t0 = time.perf_counter()
+    for i, (arr, s, seed) in enumerate(stream):
+        now = time.perf_counter() - t0
+        if arr > now:
+            time.sleep(arr - now)
+        sp, h = make_instance(J, ctx, s, seed, J.DIRECT_TABLES)
+        ev, _, st = h.solve(precision=precision, **solve_kw(J, s, sp.ndets, restol))
+        h.close()
+        sp.close()
+        out.append((i, time.perf_counter() - t0 - arr, list(ev), st))
+    return out, time.perf_counter() - t0
+
+
+def reference_energies(J, ctx, stream):
+    """Tight FP64 GPU solves (residual 1e-9, FP64 sigma) of every instance."""
+    ref = []
+    for arr, s, seed in stream:
+        sp, h = make_instance(J, ctx, s, seed, J.DIRECT_TABLES)
+        ev, _, _ = h.solve(precision=J.PREC_FP64, **solve_kw(J, s, sp.ndets, 1e-9))
+        ref.append(list(ev))
+        h.close()
+        sp.close()
+    return ref
+
+
+def bench_stream(J, ctx, args, rate, seed):
+    from jayci_amd import serve
+    stream = serve.make_stream(args.steps, rate, seed)
+    warm = serve.make_stream(args.warmup, 0.0, seed + 1) if args.warmup else []
+    ndets = [serve.ndets_of(s) for _, s, _ in stream]
+    log("stream: %d requests, %d determinants in total, largest %d"
+        % (len(stream), sum(ndets), max(ndets)))
+
+    def served(precision, st):
+        srv = serve.Server(ctx, capacity=256, budget_bytes=200e9, max_norbs=16,
+                           precision=precision, restol=1e-6)
+        try:
+            ctx.sync()
+            t0 = time.perf_counter()
+            done = srv.run(st)
+            ctx.sync()
+            wall = time.perf_counter() - t0
+            sig = srv.batch.sigma_seconds
+        finally:
+            srv.close()
+        return done, wall, sig
+
+    if warm:
+        served(J.PREC_MIXED, warm)
+        run_plain(J, ctx, warm, 1e-6, J.PREC_FP64)
+    done_m, wall_m, sig_m = served(J.PREC_MIXED, stream)
+    log("served mixed: %.3f s (sigma %.3f s)" % (wall_m, sig_m))
+    plain, wall_p = run_plain(J, ctx, stream, 1e-6, J.PREC_FP64)
+    log("plain fp64 one at a time: %.3f s" % wall_p)
+    done_b, wall_b, sig_b = served(J.PREC_FP64, stream)
+    log("served fp64: %.3f s" % wall_b)
+    ref = reference_energies(J, ctx, stream)
+    log("reference energies done")
+
+    def err_of(idx, ev):
+        return max(abs(ev[k] - ref[idx][k]) for k in range(len(ev)))
+    err_m = max(err_of(r.idx, r.energies) for r in done_m)
+    err_b = max(err_of(r.idx, r.energies) for r in done_b)
+    err_p = max(err_of(i, ev) for i, _, ev, _ in plain)
+    lat_m = [r.t_done - r.arrival for r in done_m]
+    lat_b = [r.t_done - r.arrival for r in done_b]
+    lat_p = [lat for _, lat, _, _ in plain]
+    elements = sum(sum(fci_pairs(stream[i][1])) * st["nsigma64"] for i, _, _, st in plain)
+    # CPU sample: the largest request of at most 250,000 determinants
+    small = [i for i in range(len(stream)) if ndets[i] <= 250000] or [0]
+    samp = max(small, key=lambda i: ndets[i])
+    return {
+        "value": len(done_m) / wall_m, "ms_per_step": 1e3 * wall_m / len(done_m),
+        "config": {"workload": "%s of %d full-CI requests (seed %d): n~U{8..14}, "
+                               "alpha=beta~U{3..5}, roots~U{1..4}, residual 1e-6; "
+                               "200 GB admission budget, 256 slots"
+                               % ("Poisson stream at %g requests/s" % rate if rate else
+                                  "batch present at t=0", len(stream), seed),
+                   "requests": len(stream), "determinants_total": sum(ndets),
+                   "largest_request": max(ndets), "roots_total": sum(
+                       s["nroots"] for _, s, _ in stream)},
+        "mixed": {"solves_per_s": len(done_m) / wall_m, "p50_s": percentile(lat_m, 0.5),
+                  "p99_s": percentile(lat_m, 0.99), "max_abs_err_Ha": err_m,
+                  "converged": sum(1 for r in done_m if r.stats["done"] == 1),
+                  "sigma_s": sig_m, "path": "served: continuous batching, mixed precision"},
+        "plain_fp64": {"solves_per_s": len(plain) / wall_p, "p50_s": percentile(lat_p, 0.5),
+                       "p99_s": percentile(lat_p, 0.99), "max_abs_err_Ha": err_p,
+                       "path": "one request at a time in arrival order, FP64"},
+        "served_fp64": {"solves_per_s": len(done_b) / wall_b, "p50_s": percentile(lat_b, 0.5),
+                        "p99_s": percentile(lat_b, 0.99), "max_abs_err_Ha": err_b,
+                        "sigma_s": sig_b, "path": "served: continuous batching, FP64"},
+        "mixed_over_plain": wall_p / wall_m,
+        "_cpu": (stream[samp][1], stream[samp][2], (len(plain), elements)),
+    }
+
+
+def bench_fci12(J, ctx, args):
+    from jayci_amd.workloads import WORKLOADS
+    wl = WORKLOADS["fci12"]
     m1, m2 = J.synth_integrals(wl["norbs"], wl["seed"])
     sp = J.Space(ctx, wl["na"], wl["nb"], wl["norbs"], wl["ndocc"], wl["nactv"], wl["xlvl"],
                  wl["rule"])
     t0 = time.time()
     h = J.Hamiltonian(sp, m1, m2, J.PAIRS_FP64 | J.PAIRS_FP32 | J.DIRECT_TABLES)
     setup_s = time.time() - t0
-    # autotuner: fastest sigma variant / tile width for this space becomes the default
-    best64, best32, tune_ms, width = h.autotune(nvec=1, reps=5)
+    b64, b32, tune, width = h.autotune(nvec=1, reps=5)
     N = sp.ndets
-    gold = golden_energy(args.workload)
-    solve_kw = dict(nroots=wl["nroots"], restol=1e-6, krymin=4, krymax=16, refdim=64,
-                    totfrze=J.SYNTH_ENUC, vectors=False)
+    log("setup %.2f s, autotune %s -> %d / %d (W=%d)" % (setup_s, tune, b64, b32, width))
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "cfg0_energies.json")))["energies"]
+    kw = dict(nroots=wl["nroots"], restol=1e-6, krymin=4, krymax=16, refdim=64,
+              totfrze=J.SYNTH_ENUC, vectors=False)
 
-    def log(msg):
-        print("bench: " + msg, file=sys.stderr, flush=True)
-
-    log("setup %.2f s, N=%d, pairs=%d, autotune %s -> fp64 %d fp32 %d (W=%d)"
-        % (setup_s, N, h.npairs, tune_ms, best64, best32, width))
-
-    def run(precision, k):
-        lat, errs, stats = [], [], []
+    def run(prec, k):
+        lat, err, st = [], [], None
         for _ in range(k):
             ctx.sync()
             t = time.perf_counter()
-            ev, _, st = h.solve(precision=precision, **solve_kw)
+            ev, _, st = h.solve(precision=prec, **kw)
             ctx.sync()
             lat.append(time.perf_counter() - t)
-            stats.append(st)
-            if gold:
-                errs.append(max(abs(ev[r] - gold[r]) for r in range(wl["nroots"])))
-            log("solve prec=%d %.4f s E0 %.12f iters %d" % (precision, lat[-1], ev[0],
-                                                             st["iterations"]))
-        return lat, errs, stats
-
+            err.append(max(abs(ev[r] - gold[r]) for r in range(wl["nroots"])))
+        return lat, err, st
     run(J.PREC_MIXED, args.warmup)
     run(J.PREC_FP64, args.warmup)
-    lat_m, err_m, st_m = run(J.PREC_MIXED, args.steps)
-    lat_p, err_p, st_p = run(J.PREC_FP64, args.steps)
-    tot_m, tot_p = sum(lat_m), sum(lat_p)
+    lm, em, sm = run(J.PREC_MIXED, args.steps)
+    lp, ep, spl = run(J.PREC_FP64, args.steps)
+    h.close()
+    sp.close()
+    spec = dict(na=wl["na"], nb=wl["nb"], norbs=wl["norbs"], ndocc=wl["ndocc"],
+                nactv=wl["nactv"], xlvl=wl["xlvl"], rule=wl["rule"])
+    return {
+        "value": args.steps / sum(lm), "ms_per_step": 1e3 * sum(lm) / args.steps,
+        "config": {"workload": wl["desc"], "ndets": N,
+                   "setup_s": round(setup_s, 3), "tile_width": width,
+                   "sigma_fp64": METHOD_NAMES.get(b64), "sigma_fp32": METHOD_NAMES.get(b32),
+                   "sigma_ms": {METHOD_NAMES[m]: round(v, 4) for m, v in tune.items()}},
+        "mixed": {"solves_per_s": args.steps / sum(lm), "p50_s": percentile(lm, 0.5),
+                  "p99_s": percentile(lm, 0.99), "max_abs_err_Ha": max(em),
+                  "nsigma32": sm["nsigma32"], "nsigma64": sm["nsigma64"]},
+        "plain_fp64": {"solves_per_s": args.steps / sum(lp), "p50_s": percentile(lp, 0.5),
+                       "p99_s": percentile(lp, 0.99), "max_abs_err_Ha": max(ep),
+                       "nsigma64": spl["nsigma64"]},
+        "mixed_over_plain": sum(lp) / sum(lm),
+        "_cpu": (spec, wl["seed"], (1, sum(fci_pairs(spec)) * spl["nsigma64"])),
+    }
 
-    # roofline of the dominant kernel: FP64 pair-list sigma, one vector
-    P_full = h.npairs
-    P_u = P_full // 2
-    B_sigma = P_u * (8 + 4) + N * 1 * 2 * 8 + 8 * N
-    ms = h.bench_sigma(J.SIGMA_PAIRS64, 1, 20)
-    achieved = B_sigma / (ms * 1e-3)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sigma_pairs64_fci12_traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
 
-    log("sigma pairs64 %.4f ms/launch" % ms)
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="served", choices=["served", "batched", "fci12"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (there is no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = {"served": 64, "batched": 256, "fci12": 5}[args.workload]
+    if args.warmup is None:
+        args.warmup = {"served": 4, "batched": 4, "fci12": 1}[args.workload]
+
+    import jayci_amd as J
+    ctx = J.Context(0)
+    if args.workload == "fci12":
+        res = bench_fci12(J, ctx, args)
+    elif args.workload == "served":
+        res = bench_stream(J, ctx, args, 200.0, 400)
+    else:
+        res = bench_stream(J, ctx, args, 0.0, 300)
+    sample, sseed, work = res.pop("_cpu")
+    from jayci_amd.workloads import WORKLOADS
+    wl = WORKLOADS["fci12"]
+    roof = roofline_of(J, ctx, {k: wl[k] for k in ("na", "nb", "norbs", "ndocc", "nactv",
+                                                  "xlvl", "rule")}, wl["seed"])
+    log("roofline %.1f GB/s (%.3f of peak)" % (roof["achieved"], roof["frac"]))
     cpu = None
     if not args.no_cpu:
-        nsig = st_p[0]["nsigma64"] if st_p else 0
         try:
-            cpu = cpu_baseline(wl, m1, m2, max(nsig, 1))
+            cpu = cpu_baseline(J, sample, sseed, work)
         except Exception as e:  # reported, not fatal
             cpu = {"value": None, "unit": "solves/s", "cores": None, "kind": "reference",
                    "sample": "failed: %s" % e}
-
-    line = {
-        "metric": "Davidson solves/s + p99 latency at 1e-8 Ha, mixed vs plain FP64, 1 GPU",
-        "value": args.steps / tot_m,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * tot_m / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 sigma -> f64 refinement (mixed); f64 (plain)",
-        "data": "synthetic integrals (SplitMix64/Box-Muller, seed %d)" % wl["seed"],
-        "config": {"workload": wl["desc"], "ndets": N, "pairs_full": P_full,
-                   "setup_s": round(setup_s, 3),
-                   "sigma_fp64": METHOD_NAMES.get(best64), "sigma_fp32": METHOD_NAMES.get(best32),
-                   "tile_width": width,
-                   "sigma_ms": {METHOD_NAMES[m]: round(v, 4) for m, v in tune_ms.items()}},
-        "mixed": {"solves_per_s": args.steps / tot_m, "p50_s": percentile(lat_m, 0.5),
-                  "p99_s": percentile(lat_m, 0.99),
-                  "max_abs_err_Ha": max(err_m) if err_m else None,
-                  "nsigma32": st_m[0]["nsigma32"], "nsigma64": st_m[0]["nsigma64"],
-                  "iterations": st_m[0]["iterations"]},
-        "plain_fp64": {"solves_per_s": args.steps / tot_p, "p50_s": percentile(lat_p, 0.5),
-                       "p99_s": percentile(lat_p, 0.99),
-                       "max_abs_err_Ha": max(err_p) if err_p else None,
-                       "nsigma64": st_p[0]["nsigma64"], "iterations": st_p[0]["iterations"]},
-        "mixed_over_plain": tot_p / tot_m,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": "k_spmv<double> (FP64 pair list, 1 vector)",
-                     "ms_per_launch": ms, "algorithmic_bytes": B_sigma},
-        "cpu_baseline": cpu,
-    }
-    line["roofline"]["achieved"] = achieved / 1e9
-    line["roofline"]["peak"] = PEAK_HBM / 1e9
+    line = {"metric": METRIC, "value": res["value"], "unit": "solves/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 sigma -> f64 refinement (value); f64 (plain_fp64)",
+            "data": "synthetic integrals (SplitMix64/Box-Muller), seeded",
+            "config": res["config"], "mixed": res["mixed"], "plain_fp64": res["plain_fp64"]}
+    if "served_fp64" in res:
+        line["served_fp64"] = res["served_fp64"]
+    line.update({"mixed_over_plain": res["mixed_over_plain"], "roofline": roof,
+                 "cpu_baseline": cpu})
     print(json.dumps(line))
 
 

@@ -149,6 +149,8 @@ typedef struct jci_ham jci_ham;
 #define JCI_SIGMA_DIRECT32 5
 #define JCI_SIGMA_FAST64 6  /* matrix-free, excitation tables, FP64 */
 #define JCI_SIGMA_FAST32 7  /* matrix-free, excitation tables, FP32 accumulation */
+#define JCI_SIGMA_FCI64 8   /* matrix-free, single-block (FCI) spaces, n <= 16, FP64 */
+#define JCI_SIGMA_FCI32 9   /* ... FP32 accumulation */
 
 #define JCI_PREC_FP64 0
 #define JCI_PREC_MIXED 1
@@ -237,10 +239,12 @@ int jci_bench_sigma(jci_ham *h, int method, int nvec, int reps, double *ms_per_l
 int jci_bench_copy(jci_ctx *ctx, long long bytes, int reps, double *ms_per_launch);
 
 /* Autotuner: times every sigma variant the Hamiltonian holds (pair list FP64/FP32;
- * matrix-free FP64/FP32 with beta tiles of 64, 128 and 256 strings) on nvec vectors and
+ * matrix-free FP64/FP32 with beta tiles of 64, 128 and 256 strings; the single-block
+ * kernel where it applies) on nvec vectors and
  * makes the fastest FP64 / FP32 ones what JCI_SIGMA_AUTO resolves to.  times (may be
  * NULL) gets ms per launch indexed by JCI_SIGMA_* (0 where not available). */
 int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32, double *times);
+/* times[] above has JCI_SIGMA_FCI32 + 1 entries */
 /* beta strings per tile of the matrix-free sigma (0 without excitation tables) */
 int jci_ham_fast_width(const jci_ham *h);
 

@@ -18,7 +18,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "jayci_gpu.h")
 RULE_SERIAL, RULE_MRCISD = 0, 1
 PAIRS_FP64, PAIRS_FP32, PAIRS_UPPER, DIRECT_TABLES = 1, 2, 4, 8
 (SIGMA_AUTO, SIGMA_PAIRS64, SIGMA_PAIRS32, SIGMA_PAIRS_SYM64, SIGMA_DIRECT64, SIGMA_DIRECT32,
- SIGMA_FAST64, SIGMA_FAST32) = range(8)
+ SIGMA_FAST64, SIGMA_FAST32, SIGMA_FCI64, SIGMA_FCI32) = range(10)
 PREC_FP64, PREC_MIXED = 0, 1
 GUESS_BLOCK, GUESS_USER, GUESS_DIAG = 1, 2, 3
 MAXROOTS = 8
@@ -325,10 +325,10 @@ class Hamiltonian:
     def autotune(self, nvec=1, reps=5):
         """Time all sigma variants; returns (best64, best32, {method: ms}, tile width)."""
         b64, b32 = _i(), _i()
-        t = np.zeros(8)
+        t = np.zeros(10)
         _check(_L.jci_ham_autotune(self._h, nvec, reps, ctypes.byref(b64), ctypes.byref(b32), _p(t)),
                "jci_ham_autotune")
-        return b64.value, b32.value, {m: t[m] for m in range(8) if t[m] > 0}, \
+        return b64.value, b32.value, {m: t[m] for m in range(10) if t[m] > 0}, \
             _L.jci_ham_fast_width(self._h)
 
     def sigma(self, x, method=SIGMA_AUTO):

@@ -252,6 +252,7 @@ struct jci_ham {
     // excitation tables of the matrix-free sigma (spin 1 aliases spin 0 when equal)
     int have_fast;
     long long *d_off1[2], *d_off2[2];
+    int *d_goff1[2], *d_goff2[2];
     int2 *d_e1[2], *d_e2[2];
     double *d_v1[2], *d_v2[2];
     long long nexc1[2], nexc2[2];
@@ -259,6 +260,8 @@ struct jci_ham {
     long long *d_tile_off[3];   // tile maps for W = 64, 128, 256
     FastArgs fast[3];
     int fast_w;                 // index into fast[] used by the FAST methods
+    int have_fci;               // single-block space with n <= 16: FCI kernel usable
+    FciArgs fci;
     int tuned64, tuned32;       // autotuned default methods (0 = not tuned)
 };
 
@@ -314,11 +317,15 @@ static int build_fast(jci_ham *h, int W)
     for (int spin = 0; spin < (same ? 1 : 2); spin++) {
         long long n = D.s[spin].gstart[D.s[spin].ngrp];
         long long *c1, *c2;
+        int G = D.s[spin].ngrp;
         JCI_CHECK(hipMalloc(&c1, sizeof(long long) * (n + 1)));
         JCI_CHECK(hipMalloc(&c2, sizeof(long long) * (n + 1)));
         JCI_CHECK(hipMalloc(&h->d_off1[spin], sizeof(long long) * (n + 1)));
         JCI_CHECK(hipMalloc(&h->d_off2[spin], sizeof(long long) * (n + 1)));
-        RCHECK(jk_exc_count(s->d_desc, spin, n, s->d_mask[spin], c->d_binom, c1, c2, c->stream));
+        JCI_CHECK(hipMalloc(&h->d_goff1[spin], sizeof(int) * n * (G + 1)));
+        JCI_CHECK(hipMalloc(&h->d_goff2[spin], sizeof(int) * n * (G + 1)));
+        RCHECK(jk_exc_count(s->d_desc, spin, n, s->d_mask[spin], c->d_binom, c1, c2,
+                            h->d_goff1[spin], h->d_goff2[spin], c->stream));
         RCHECK(jk_scan(c1, h->d_off1[spin], n, c->stream));
         RCHECK(jk_scan(c2, h->d_off2[spin], n, c->stream));
         JCI_CHECK(hipMemcpyAsync(&h->nexc1[spin], h->d_off1[spin] + n, sizeof(long long),
@@ -338,11 +345,12 @@ static int build_fast(jci_ham *h, int W)
             return -3;
         }
         RCHECK(jk_exc_fill(s->d_desc, spin, n, s->d_mask[spin], c->d_binom, h->d_m1, h->d_m2,
-                           h->d_off1[spin], h->d_off2[spin], h->d_e1[spin], h->d_v1[spin],
-                           h->d_e2[spin], h->d_v2[spin], c->stream));
+                           h->d_off1[spin], h->d_off2[spin], h->d_goff1[spin], h->d_goff2[spin],
+                           h->d_e1[spin], h->d_v1[spin], h->d_e2[spin], h->d_v2[spin], c->stream));
     }
     if (same) {
         h->d_off1[1] = h->d_off1[0]; h->d_off2[1] = h->d_off2[0];
+        h->d_goff1[1] = h->d_goff1[0]; h->d_goff2[1] = h->d_goff2[0];
         h->d_e1[1] = h->d_e1[0]; h->d_e2[1] = h->d_e2[0];
         h->d_v1[1] = h->d_v1[0]; h->d_v2[1] = h->d_v2[0];
         h->nexc1[1] = h->nexc1[0]; h->nexc2[1] = h->nexc2[0];
@@ -366,8 +374,10 @@ static int build_fast(jci_ham *h, int W)
         a.d = s->d_desc;
         a.ma = s->d_mask[0];
         a.mb = s->d_mask[1];
-        a.ta = ExcView{h->d_off1[0], h->d_off2[0], h->d_e1[0], h->d_e2[0], h->d_v1[0], h->d_v2[0]};
-        a.tb = ExcView{h->d_off1[1], h->d_off2[1], h->d_e1[1], h->d_e2[1], h->d_v1[1], h->d_v2[1]};
+        a.ta = ExcView{h->d_off1[0], h->d_off2[0], h->d_e1[0], h->d_e2[0], h->d_v1[0], h->d_v2[0],
+                       h->d_goff1[0], h->d_goff2[0], D.s[0].ngrp};
+        a.tb = ExcView{h->d_off1[1], h->d_off2[1], h->d_e1[1], h->d_e2[1], h->d_v1[1], h->d_v2[1],
+                       h->d_goff1[1], h->d_goff2[1], D.s[1].ngrp};
         a.m2sq = h->d_m2sq;
         a.hd = h->d_hd;
         a.tile_off = h->d_tile_off[wi];
@@ -377,6 +387,22 @@ static int build_fast(jci_ham *h, int W)
         if (Wd == W) h->fast_w = wi;
     }
     h->have_fast = 1;
+    if (D.nblk == 1 && D.norbs <= FCI_MAXORB) {
+        FciArgs &f = h->fci;
+        f.ma = s->d_mask[0];
+        f.mb = s->d_mask[1];
+        f.ta = h->fast[0].ta;
+        f.tb = h->fast[0].tb;
+        f.m2 = h->d_m2;
+        f.hd = h->d_hd;
+        f.na = D.s[0].gn[D.bp[0]];
+        f.nb = D.s[1].gn[D.bq[0]];
+        f.m1 = mp;
+        f.m2len = mp * (mp + 1) / 2;
+        f.m2len_pad = (f.m2len + 3) / 4 * 4;
+        long long maxs = (long long)std::max(s->na, s->nb) * (D.norbs - std::min(s->na, s->nb));
+        h->have_fci = (D.s[0].gstart[0] == 0 && D.s[1].gstart[0] == 0 && maxs <= FCI_MAXSINGLES);
+    }
     return 0;
 }
 
@@ -421,6 +447,7 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     for (int sp = 0; sp < 2; sp++) {
         if (sp == 1 && h->d_off1[1] == h->d_off1[0]) break;
         hipFree(h->d_off1[sp]); hipFree(h->d_off2[sp]); hipFree(h->d_e1[sp]);
+        hipFree(h->d_goff1[sp]); hipFree(h->d_goff2[sp]);
         hipFree(h->d_e2[sp]); hipFree(h->d_v1[sp]); hipFree(h->d_v2[sp]);
     }
     hipFree(h->d_m2sq);
@@ -481,9 +508,11 @@ static int resolve_method(const jci_ham *h, int method, int want32)
     if (!want32 && h->tuned64) return h->tuned64;
     if (want32) {
         if (h->d_val32) return JCI_SIGMA_PAIRS32;
+        if (h->have_fci) return JCI_SIGMA_FCI32;
         return h->have_fast ? JCI_SIGMA_FAST32 : JCI_SIGMA_DIRECT32;
     }
     if (h->d_val64) return h->upper ? JCI_SIGMA_PAIRS_SYM64 : JCI_SIGMA_PAIRS64;
+    if (h->have_fci) return JCI_SIGMA_FCI64;
     return h->have_fast ? JCI_SIGMA_FAST64 : JCI_SIGMA_DIRECT64;
 }
 
@@ -540,6 +569,14 @@ static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *
         RCHECK(ensure_xf(h, N * nvec));
         RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
         return jk_sigma_fast(32, nvec, h->fast[h->fast_w], h->d_xf, y, c->stream);
+    case JCI_SIGMA_FCI64:
+        if (!h->have_fci) { jci_set_error("sigma: single-block kernel not applicable"); return -1; }
+        return jk_sigma_fci(64, nvec, h->fci, x, y, c->stream);
+    case JCI_SIGMA_FCI32:
+        if (!h->have_fci) { jci_set_error("sigma: single-block kernel not applicable"); return -1; }
+        RCHECK(ensure_xf(h, N * nvec));
+        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
+        return jk_sigma_fci(32, nvec, h->fci, h->d_xf, y, c->stream);
     default:
         jci_set_error("sigma: unknown method %d", method);
         return -1;
@@ -686,7 +723,7 @@ JCI_API int jci_bench_sigma(jci_ham *h, int method, int nvec, int reps, double *
 JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32,
                              double *times)
 {
-    double t[8] = {0};
+    double t[10] = {0};
     double best_w_ms = 1e300;
     int best_w = h->fast_w;
     if (h->have_fast) {
@@ -700,6 +737,10 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
         t[JCI_SIGMA_FAST64] = best_w_ms;
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST32, nvec, reps, &t[JCI_SIGMA_FAST32]));
     }
+    if (h->have_fci) {
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_FCI64, nvec, reps, &t[JCI_SIGMA_FCI64]));
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_FCI32, nvec, reps, &t[JCI_SIGMA_FCI32]));
+    }
     if (h->d_val64 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS64, nvec, reps, &t[JCI_SIGMA_PAIRS64]));
     if (h->d_val32 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS32, nvec, reps, &t[JCI_SIGMA_PAIRS32]));
     if (!h->have_fast && !h->d_val64 && !h->d_val32) {
@@ -707,15 +748,15 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_DIRECT32, nvec, 1, &t[JCI_SIGMA_DIRECT32]));
     }
     int b64 = 0, b32 = 0;
-    const int c64[] = {JCI_SIGMA_PAIRS64, JCI_SIGMA_FAST64, JCI_SIGMA_DIRECT64};
-    const int c32[] = {JCI_SIGMA_PAIRS32, JCI_SIGMA_FAST32, JCI_SIGMA_DIRECT32};
+    const int c64[] = {JCI_SIGMA_PAIRS64, JCI_SIGMA_FAST64, JCI_SIGMA_DIRECT64, JCI_SIGMA_FCI64};
+    const int c32[] = {JCI_SIGMA_PAIRS32, JCI_SIGMA_FAST32, JCI_SIGMA_DIRECT32, JCI_SIGMA_FCI32};
     for (int m : c64) if (t[m] > 0 && (b64 == 0 || t[m] < t[b64])) b64 = m;
     for (int m : c32) if (t[m] > 0 && (b32 == 0 || t[m] < t[b32])) b32 = m;
     h->tuned64 = b64;
     h->tuned32 = b32;
     if (best64) *best64 = b64;
     if (best32) *best32 = b32;
-    if (times) for (int m = 0; m < 8; m++) times[m] = t[m];
+    if (times) for (int m = 0; m < 10; m++) times[m] = t[m];
     return 0;
 }
 

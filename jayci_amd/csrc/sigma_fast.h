@@ -3,10 +3,14 @@
 #include "jci_internal.h"
 
 // device view of one string set's excitation tables (see sigma_fast.hip)
+// Entries of a string are grouped by target group: the entries of string I that go to
+// group g are [off[I] + goff[I*(G+1) + g], off[I] + goff[I*(G+1) + g + 1]).
 struct ExcView {
     const long long *off1, *off2;
     const int2 *e1, *e2;
     const double *v1, *v2;
+    const int *goff1, *goff2;
+    int G;
 };
 
 // kernel arguments of one space
@@ -22,13 +26,26 @@ struct FastArgs {
     long long ntiles;
 };
 
+// single-block (FCI) specialisation
+#define FCI_MAXSINGLES 256   // alpha singles per string: na * (n - na) <= 256
+#define FCI_MAXORB 16        // packed integrals in LDS: n <= 16
+struct FciArgs {
+    const uint64_t *ma, *mb;
+    ExcView ta, tb;
+    const double *m2;            // packed (ij|kl)
+    const double *hd;
+    long long na, nb;            // alpha / beta strings
+    int m1, m2len, m2len_pad;
+};
+int jk_sigma_fci(int prec, int nvec, const FciArgs &a, const void *x, double *y, hipStream_t st);
+
 int jk_exc_count(const SpaceDesc *d, int spin, long long nstr, const uint64_t *M,
                  const unsigned long long *binom, long long *cnt1, long long *cnt2,
-                 hipStream_t st);
+                 int *goff1, int *goff2, hipStream_t st);
 int jk_exc_fill(const SpaceDesc *d, int spin, long long nstr, const uint64_t *M,
                 const unsigned long long *binom, const double *m1, const double *m2,
-                const long long *off1, const long long *off2, int2 *e1, double *v1, int2 *e2,
-                double *v2, hipStream_t st);
+                const long long *off1, const long long *off2, const int *goff1,
+                const int *goff2, int2 *e1, double *v1, int2 *e2, double *v2, hipStream_t st);
 int jk_m2_square(int m1, const double *m2, double *sq, hipStream_t st);
 int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *y,
                   hipStream_t st);

@@ -56,19 +56,32 @@ __device__ __forceinline__ long long str_index2(const SpaceDesc *d, int s, uint6
 
 // ---- excitation tables -------------------------------------------------------------------
 
+// Counting pass: per-string totals and per-target-group offsets (goff rows of G+1);
+// fill pass: entries written grouped by target group, in (i, f) enumeration order inside
+// a group.
 template <bool FILL>
 __global__ void k_exc(const SpaceDesc *__restrict__ d, int spin, long long nstr,
                       const uint64_t *__restrict__ M, const unsigned long long *__restrict__ binom,
                       const double *__restrict__ m1, const double *__restrict__ m2,
                       long long *__restrict__ cnt1, long long *__restrict__ cnt2,
+                      int *__restrict__ goff1, int *__restrict__ goff2,
                       const long long *__restrict__ off1, const long long *__restrict__ off2,
                       int2 *__restrict__ e1, double *__restrict__ v1, int2 *__restrict__ e2,
                       double *__restrict__ v2)
 {
     long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nstr) return;
+    const int G = d->s[spin].ngrp;
     const uint64_t m = M[I], vir = d->allmask & ~m;
-    long long n1 = 0, n2 = 0, p1 = FILL ? off1[I] : 0, p2 = FILL ? off2[I] : 0;
+    int c1[JCI_MAXG + 1], c2[JCI_MAXG + 1];
+    for (int g = 0; g <= G; g++) { c1[g] = 0; c2[g] = 0; }
+    if (FILL) {
+        for (int g = 0; g <= G; g++) {
+            c1[g] = goff1[I * (G + 1) + g];
+            c2[g] = goff2[I * (G + 1) + g];
+        }
+    }
+    const long long b1 = FILL ? off1[I] : 0, b2 = FILL ? off2[I] : 0;
     for (uint64_t x = m; x; x &= x - 1) {
         int i = jci_ctz(x);
         for (uint64_t y = vir; y; y &= y - 1) {
@@ -77,6 +90,7 @@ __global__ void k_exc(const SpaceDesc *__restrict__ d, int spin, long long nstr,
             long long J = str_index2(d, spin, m ^ (1ULL << i) ^ (1ULL << f), binom, &g);
             if (J < 0) continue;
             if (FILL) {
+                long long p1 = b1 + c1[g];
                 int neg = jci_sign(m, i, f) < 0;
                 e1[p1] = make_int2((int)J, i | (f << 6) | (neg << 12) | (g << 13));
                 double v = jci_h1(m1, i, f);
@@ -85,9 +99,8 @@ __global__ void k_exc(const SpaceDesc *__restrict__ d, int spin, long long nstr,
                     v += jci_eri(m2, k, k, i, f) - jci_eri(m2, k, i, k, f);
                 }
                 v1[p1] = v;
-                p1++;
             }
-            n1++;
+            c1[g]++;
         }
     }
     for (uint64_t x0 = m; x0; x0 &= x0 - 1) {
@@ -104,40 +117,53 @@ __global__ void k_exc(const SpaceDesc *__restrict__ d, int spin, long long nstr,
                                              binom, &g);
                     if (J < 0) continue;
                     if (FILL) {
+                        long long p2 = b2 + c2[g];
                         int ph;
                         double v = jci_double_same(m, i0, i1, f0, f1, m2, &ph);
                         e2[p2] = make_int2((int)J, i0 | (i1 << 6) | (f0 << 12) | (f1 << 18) |
                                                        ((ph < 0) << 24) | (g << 25));
                         v2[p2] = v;
-                        p2++;
                     }
-                    n2++;
+                    c2[g]++;
                 }
             }
         }
     }
-    if (!FILL) { cnt1[I] = n1; cnt2[I] = n2; }
+    if (!FILL) {
+        long long t1 = 0, t2 = 0;
+        for (int g = 0; g < G; g++) {
+            goff1[I * (G + 1) + g] = (int)t1;
+            goff2[I * (G + 1) + g] = (int)t2;
+            t1 += c1[g];
+            t2 += c2[g];
+        }
+        goff1[I * (G + 1) + G] = (int)t1;
+        goff2[I * (G + 1) + G] = (int)t2;
+        cnt1[I] = t1;
+        cnt2[I] = t2;
+    }
 }
 
 int jk_exc_count(const SpaceDesc *d, int spin, long long nstr, const uint64_t *M,
-                 const unsigned long long *binom, long long *cnt1, long long *cnt2, hipStream_t st)
+                 const unsigned long long *binom, long long *cnt1, long long *cnt2,
+                 int *goff1, int *goff2, hipStream_t st)
 {
     hipLaunchKernelGGL(k_exc<false>, dim3(gridn(nstr, 64)), dim3(64), 0, st, d, spin, nstr, M,
                        binom, (const double *)nullptr, (const double *)nullptr, cnt1, cnt2,
-                       (const long long *)nullptr, (const long long *)nullptr, (int2 *)nullptr,
-                       (double *)nullptr, (int2 *)nullptr, (double *)nullptr);
+                       goff1, goff2, (const long long *)nullptr, (const long long *)nullptr,
+                       (int2 *)nullptr, (double *)nullptr, (int2 *)nullptr, (double *)nullptr);
     LAUNCH_CHECK2();
     return 0;
 }
 
 int jk_exc_fill(const SpaceDesc *d, int spin, long long nstr, const uint64_t *M,
                 const unsigned long long *binom, const double *m1, const double *m2,
-                const long long *off1, const long long *off2, int2 *e1, double *v1, int2 *e2,
-                double *v2, hipStream_t st)
+                const long long *off1, const long long *off2, const int *goff1,
+                const int *goff2, int2 *e1, double *v1, int2 *e2, double *v2, hipStream_t st)
 {
     hipLaunchKernelGGL(k_exc<true>, dim3(gridn(nstr, 64)), dim3(64), 0, st, d, spin, nstr, M,
-                       binom, m1, m2, (long long *)nullptr, (long long *)nullptr, off1, off2,
-                       e1, v1, e2, v2);
+                       binom, m1, m2, (long long *)nullptr, (long long *)nullptr,
+                       (int *)goff1, (int *)goff2, off1, off2, e1, v1, e2, v2);
     LAUNCH_CHECK2();
     return 0;
 }
@@ -172,11 +198,31 @@ __device__ __forceinline__ int tile_block(const long long *__restrict__ toff, in
     return lo;
 }
 
+// space lookups cached in LDS by every work-group
+struct SpaceLds {
+    int blkid[JCI_MAXG][JCI_MAXG];
+    long long boff[JCI_MAXB + 1];
+    long long gsA[JCI_MAXG + 1], gnA[JCI_MAXG], gsB[JCI_MAXG + 1], gnB[JCI_MAXG];
+    int GA, GB;
+};
+
+__device__ __forceinline__ void load_space_lds(const SpaceDesc *__restrict__ d, SpaceLds *L, int tid, int nt)
+{
+    for (int t = tid; t < JCI_MAXG * JCI_MAXG; t += nt) L->blkid[t / JCI_MAXG][t % JCI_MAXG] = d->blkid[t / JCI_MAXG][t % JCI_MAXG];
+    for (int t = tid; t <= d->nblk; t += nt) L->boff[t] = d->boff[t];
+    for (int t = tid; t < JCI_MAXG; t += nt) {
+        L->gsA[t] = d->s[0].gstart[t]; L->gnA[t] = d->s[0].gn[t];
+        L->gsB[t] = d->s[1].gstart[t]; L->gnB[t] = d->s[1].gn[t];
+    }
+    if (tid == 0) { L->GA = d->s[0].ngrp; L->GB = d->s[1].ngrp; }
+}
+
 template <typename AT, typename XT, int K, int W>
 __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restrict__ x,
                                                   double *__restrict__ y)
 {
     extern __shared__ unsigned char smem_raw[];
+    __shared__ SpaceLds L;
     AT *g = (AT *)smem_raw;              // staged integral row (i f | .), m1 entries
     AT *Fa = g + a.m1;                   // sum_{k in Ia} (kk | .), m1 entries
     const SpaceDesc *__restrict__ d = a.d;
@@ -195,6 +241,7 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     const uint64_t ma = a.ma[Ia];
     const uint64_t mb = a.mb[Ib];
     const long long I = d->boff[b] + ia_loc * nq + ib_loc;
+    load_space_lds(d, &L, tid, W);
 
     // Fa[P] = sum over alpha electrons k of (kk|P)
     for (int P = tid; P < a.m1; P += W) {
@@ -216,49 +263,55 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
 #pragma unroll
         for (int k = 0; k < K; k++) acc[k] = (AT)0;
     }
-    const SpinDesc &SA = d->s[0], &SB = d->s[1];
     const ExcView &TA = a.ta, &TB = a.tb;
+    const int GA = L.GA, GB = L.GB;
 
-    // beta excitations of Ib (per thread)
+    // beta excitations of Ib (per thread), only towards groups paired with p
     if (active) {
-        for (long long e = TB.off1[Ib]; e < TB.off1[Ib + 1]; e++) {
-            int2 en = TB.e1[e];
-            int gq = (en.y >> 13) & 31;
-            int bk = d->blkid[p][gq];
+        const long long o1 = TB.off1[Ib], o2 = TB.off2[Ib];
+        const int *go1 = TB.goff1 + Ib * (GB + 1), *go2 = TB.goff2 + Ib * (GB + 1);
+        for (int gq = 0; gq < GB; gq++) {
+            int bk = L.blkid[p][gq];
             if (bk < 0) continue;
-            int i = en.y & 63, f = (en.y >> 6) & 63;
-            long long J = d->boff[bk] + ia_loc * SB.gn[gq] + (en.x - SB.gstart[gq]);
-            AT v = (AT)TB.v1[e] + Fa[jci_tri(i, f)];
-            if (en.y & (1 << 12)) v = -v;
+            const long long base = L.boff[bk] + ia_loc * L.gnB[gq] - L.gsB[gq];
+            for (long long e = o1 + go1[gq]; e < o1 + go1[gq + 1]; e++) {
+                int2 en = TB.e1[e];
+                int i = en.y & 63, f = (en.y >> 6) & 63;
+                AT v = (AT)TB.v1[e] + Fa[jci_tri(i, f)];
+                if (en.y & (1 << 12)) v = -v;
+                long long J = base + en.x;
 #pragma unroll
-            for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
-        }
-        for (long long e = TB.off2[Ib]; e < TB.off2[Ib + 1]; e++) {
-            int2 en = TB.e2[e];
-            int gq = (en.y >> 25) & 31;
-            int bk = d->blkid[p][gq];
-            if (bk < 0) continue;
-            long long J = d->boff[bk] + ia_loc * SB.gn[gq] + (en.x - SB.gstart[gq]);
-            AT v = (AT)TB.v2[e];
+                for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+            }
+            for (long long e = o2 + go2[gq]; e < o2 + go2[gq + 1]; e++) {
+                AT v = (AT)TB.v2[e];
+                long long J = base + TB.e2[e].x;
 #pragma unroll
-            for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+                for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+            }
         }
     }
 
-    // alpha doubles (uniform over the work-group)
-    for (long long e = TA.off2[Ia]; e < TA.off2[Ia + 1]; e++) {
-        int2 en = TA.e2[e];
-        int gp = (en.y >> 25) & 31;
-        int bk = d->blkid[gp][q];
-        if (bk < 0 || !active) continue;
-        long long J = d->boff[bk] + (en.x - SA.gstart[gp]) * nq + ib_loc;
-        AT v = (AT)TA.v2[e];
+    // alpha doubles (uniform over the work-group), only towards groups paired with q
+    {
+        const long long o2 = TA.off2[Ia];
+        const int *go2 = TA.goff2 + Ia * (GA + 1);
+        for (int gp = 0; gp < GA; gp++) {
+            int bk = L.blkid[gp][q];
+            if (bk < 0 || !active) continue;
+            const long long base = L.boff[bk] - L.gsA[gp] * nq + ib_loc;
+            for (long long e = o2 + go2[gp]; e < o2 + go2[gp + 1]; e++) {
+                AT v = (AT)TA.v2[e];
+                long long J = base + (long long)TA.e2[e].x * nq;
 #pragma unroll
-        for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+                for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+            }
+        }
     }
 
     // alpha singles (uniform): own element and products with beta singles
-    const long long b1 = TB.off1[Ib], e1end = TB.off1[Ib + 1];
+    const long long ob1 = TB.off1[Ib];
+    const int *gob1 = TB.goff1 + Ib * (GB + 1);
     for (long long e = TA.off1[Ia]; e < TA.off1[Ia + 1]; e++) {
         int2 en = TA.e1[e];
         int i = en.y & 63, f = (en.y >> 6) & 63, gp = (en.y >> 13) & 31;
@@ -268,8 +321,8 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
         for (int Q = tid; Q < a.m1; Q += W) g[Q] = (AT)a.m2sq[P * a.m1 + Q];
         __syncthreads();
         if (!active) continue;
-        long long ja_loc = en.x - SA.gstart[gp];
-        int bk = d->blkid[gp][q];
+        long long ja_loc = en.x - L.gsA[gp];
+        int bk = L.blkid[gp][q];
         if (bk >= 0) {
             AT cross = (AT)0;
             for (uint64_t z = mb; z; z &= z - 1) {
@@ -278,21 +331,23 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
             }
             AT v = (AT)TA.v1[e] + cross;
             if (neg) v = -v;
-            long long J = d->boff[bk] + ja_loc * nq + ib_loc;
+            long long J = L.boff[bk] + ja_loc * nq + ib_loc;
 #pragma unroll
             for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
         }
-        for (long long e2 = b1; e2 < e1end; e2++) {
-            int2 eb = TB.e1[e2];
-            int gq = (eb.y >> 13) & 31;
-            int bk2 = d->blkid[gp][gq];
+        for (int gq = 0; gq < GB; gq++) {
+            int bk2 = L.blkid[gp][gq];
             if (bk2 < 0) continue;
-            int kk = eb.y & 63, ll = (eb.y >> 6) & 63;
-            AT v = g[jci_tri(kk, ll)];
-            if (neg != (bool)(eb.y & (1 << 12))) v = -v;
-            long long J = d->boff[bk2] + ja_loc * SB.gn[gq] + (eb.x - SB.gstart[gq]);
+            const long long base = L.boff[bk2] + ja_loc * L.gnB[gq] - L.gsB[gq];
+            for (long long e2 = ob1 + gob1[gq]; e2 < ob1 + gob1[gq + 1]; e2++) {
+                int2 eb = TB.e1[e2];
+                int kk = eb.y & 63, ll = (eb.y >> 6) & 63;
+                AT v = g[jci_tri(kk, ll)];
+                if (neg != (bool)(eb.y & (1 << 12))) v = -v;
+                long long J = base + eb.x;
 #pragma unroll
-            for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+                for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+            }
         }
     }
     if (active) {
@@ -334,4 +389,157 @@ int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *
     }
     if (prec == 64) return launch_fast<double, double, 256>(a, nvec, (const double *)x, y, st);
     return launch_fast<float, float, 256>(a, nvec, (const float *)x, y, st);
+}
+
+// ---- single-block spaces (FCI) ------------------------------------------------------------
+// One work-group of 256 threads owns (Ia, Ib_0 .. Ib_0+255).  The packed two-electron
+// integrals live in LDS (n <= 16 orbitals), the work-group's alpha singles are decoded
+// once into LDS, determinant addresses are Ja * Nb + Jb.  The alpha-beta doubles run with
+// the lane's beta singles outside and the uniform alpha singles inside, so the hot loop
+// has no barrier and reads no table from global memory.
+struct FciAlpha {
+    int ja;      // target alpha string
+    int P;       // pair index tri(i, f)
+    int Ptri;    // P(P+1)/2
+    int neg;     // sign bit
+};
+
+template <typename AT, typename XT, int K>
+__global__ __launch_bounds__(256) void k_sigma_fci(FciArgs a, const XT *__restrict__ x,
+                                                   double *__restrict__ y)
+{
+    extern __shared__ unsigned char smem_f[];
+    AT *m2s = (AT *)smem_f;
+    FciAlpha *al = (FciAlpha *)(m2s + a.m2len_pad);
+    AT *alv = (AT *)(al + FCI_MAXSINGLES);
+    AT *Fa = alv + FCI_MAXSINGLES;
+    const int tid = threadIdx.x;
+    const long long Nb = a.nb;
+    const long long nch = (Nb + 255) / 256;
+    const long long Ia = blockIdx.x / nch;
+    const long long ib = (blockIdx.x - Ia * nch) * 256 + tid;
+    const bool active = ib < Nb;
+    const uint64_t ma = a.ma[Ia];
+    const uint64_t mb = a.mb[active ? ib : 0];
+    for (int t = tid; t < a.m2len; t += 256) m2s[t] = (AT)a.m2[t];
+    const long long s0 = a.ta.off1[Ia];
+    const int nsa = (int)(a.ta.off1[Ia + 1] - s0);
+    for (int t = tid; t < nsa; t += 256) {
+        int2 en = a.ta.e1[s0 + t];
+        int i = en.y & 63, f = (en.y >> 6) & 63;
+        int P = (int)jci_tri(i, f);
+        al[t] = FciAlpha{en.x, P, P * (P + 1) / 2, (en.y >> 12) & 1};
+        alv[t] = (AT)a.ta.v1[s0 + t];
+    }
+    __syncthreads();
+    for (int P = tid; P < a.m1; P += 256) {
+        AT s = (AT)0;
+        for (uint64_t z = ma; z; z &= z - 1) {
+            int k = jci_ctz(z);
+            int Q = k * (k + 1) / 2 + k;
+            s += m2s[jci_tri(Q, P)];
+        }
+        Fa[P] = s;
+    }
+    __syncthreads();
+    if (!active) return;
+    const long long I = Ia * Nb + ib;
+    AT acc[K];
+    {
+        AT h = (AT)a.hd[I];
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = h * (AT)x[I * K + k];
+    }
+    const long long rowI = Ia * Nb;
+    // beta singles and doubles of ib
+    const long long b1s = a.tb.off1[ib], b1e = a.tb.off1[ib + 1];
+    for (long long e = b1s; e < b1e; e++) {
+        int2 en = a.tb.e1[e];
+        int i = en.y & 63, f = (en.y >> 6) & 63;
+        AT v = (AT)a.tb.v1[e] + Fa[jci_tri(i, f)];
+        if (en.y & (1 << 12)) v = -v;
+        long long J = rowI + en.x;
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+    }
+    for (long long e = a.tb.off2[ib]; e < a.tb.off2[ib + 1]; e++) {
+        AT v = (AT)a.tb.v2[e];
+        long long J = rowI + a.tb.e2[e].x;
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+    }
+    // alpha doubles (uniform)
+    for (long long e = a.ta.off2[Ia]; e < a.ta.off2[Ia + 1]; e++) {
+        AT v = (AT)a.ta.v2[e];
+        long long J = (long long)a.ta.e2[e].x * Nb + ib;
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+    }
+    // alpha singles: own element (cross-spin term over the beta electrons)
+    int occb[16];
+    int nob = 0;
+    for (uint64_t z = mb; z; z &= z - 1) {
+        int k = jci_ctz(z);
+        occb[nob++] = k * (k + 1) / 2 + k;
+    }
+    for (int t = 0; t < nsa; t++) {
+        FciAlpha A = al[t];
+        AT cross = (AT)0;
+        for (int u = 0; u < nob; u++) {
+            int Q = occb[u];
+            cross += m2s[Q >= A.P ? Q * (Q + 1) / 2 + A.P : A.Ptri + Q];
+        }
+        AT v = alv[t] + cross;
+        if (A.neg) v = -v;
+        long long J = (long long)A.ja * Nb + ib;
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+    }
+    // alpha-beta doubles
+    for (long long e = b1s; e < b1e; e++) {
+        int2 en = a.tb.e1[e];
+        int i = en.y & 63, f = (en.y >> 6) & 63;
+        int Pb = (int)jci_tri(i, f);
+        int Pbtri = Pb * (Pb + 1) / 2;
+        int nb_ = (en.y >> 12) & 1;
+        long long jb = en.x;
+        for (int t = 0; t < nsa; t++) {
+            FciAlpha A = al[t];
+            AT v = m2s[A.P >= Pb ? A.Ptri + Pb : Pbtri + A.P];
+            if (A.neg ^ nb_) v = -v;
+            long long J = (long long)A.ja * Nb + jb;
+#pragma unroll
+            for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) y[I * K + k] = (double)acc[k];
+}
+
+template <typename AT, typename XT>
+static int launch_fci(const FciArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
+{
+    size_t shm = sizeof(AT) * (size_t)a.m2len_pad + sizeof(FciAlpha) * FCI_MAXSINGLES +
+                 sizeof(AT) * (FCI_MAXSINGLES + (size_t)a.m1);
+    long long nch = (a.nb + 255) / 256;
+    dim3 g((unsigned)(a.na * nch)), b(256);
+    switch (nvec) {
+    case 1: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 1>), g, b, shm, st, a, x, y); break;
+    case 2: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 2>), g, b, shm, st, a, x, y); break;
+    case 3: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 3>), g, b, shm, st, a, x, y); break;
+    case 4: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 4>), g, b, shm, st, a, x, y); break;
+    case 5: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 5>), g, b, shm, st, a, x, y); break;
+    case 6: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 6>), g, b, shm, st, a, x, y); break;
+    case 7: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 7>), g, b, shm, st, a, x, y); break;
+    case 8: hipLaunchKernelGGL((k_sigma_fci<AT, XT, 8>), g, b, shm, st, a, x, y); break;
+    default: jci_set_error("sigma_fci: nvec must be 1..8"); return -1;
+    }
+    LAUNCH_CHECK2();
+    return 0;
+}
+
+int jk_sigma_fci(int prec, int nvec, const FciArgs &a, const void *x, double *y, hipStream_t st)
+{
+    if (prec == 64) return launch_fci<double, double>(a, nvec, (const double *)x, y, st);
+    return launch_fci<float, float>(a, nvec, (const float *)x, y, st);
 }

@@ -59,8 +59,8 @@ def test_rows_bitexact(J, orc, ctx, spec):
         np.testing.assert_allclose(v, ov, rtol=1e-12, atol=1e-13)
 
 
-METHODS64 = ["SIGMA_PAIRS64", "SIGMA_PAIRS_SYM64", "SIGMA_DIRECT64", "SIGMA_FAST64"]
-METHODS32 = ["SIGMA_PAIRS32", "SIGMA_DIRECT32", "SIGMA_FAST32"]
+METHODS64 = ["SIGMA_PAIRS64", "SIGMA_PAIRS_SYM64", "SIGMA_DIRECT64", "SIGMA_FAST64", "SIGMA_FCI64"]
+METHODS32 = ["SIGMA_PAIRS32", "SIGMA_DIRECT32", "SIGMA_FAST32", "SIGMA_FCI32"]
 
 
 @pytest.mark.parametrize("spec", SMALL_SPACES)
@@ -72,13 +72,18 @@ def test_sigma_all_methods(J, orc, ctx, spec):
     X = rand_vectors(N, 3, 5)
     ref = os_.sigma(m1, m2, X)
     scale = np.abs(ref).max()
+    fci = spec[3] == 0 and spec[4] == spec[2]
     for name in METHODS64:
+        if name == "SIGMA_FCI64" and not fci:
+            continue
         meth = getattr(J, name)
         h = upper if name == "SIGMA_PAIRS_SYM64" else full
         k = 1 if name == "SIGMA_PAIRS_SYM64" else 3
         y = h.sigma(X[:k], method=meth)
         np.testing.assert_allclose(y, ref[:k], rtol=0, atol=1e-12 * scale, err_msg=name)
     for name in METHODS32:
+        if name == "SIGMA_FCI32" and not fci:
+            continue
         y = full.sigma(X, method=getattr(J, name))
         np.testing.assert_allclose(y, ref, rtol=0, atol=2e-5 * scale, err_msg=name)
 
