@@ -18,8 +18,9 @@ Workloads (synthetic integrals from seeds, resident on the GPU before each solve
 
 Each root of every path is compared with a tight FP64 GPU solve of the same instance
 (residual 1e-9, FP64 sigma); max |dE| is reported and the target is <= 1e-8 Ha.  The
-roofline is that of the plain sigma kernel (the FP64 pair list, k_spmv<double>) on the
-full-CI 4a4b/12 space, timed with HIP events on the library's stream.  The CPU baseline
+roofline is that of the dominant kernel, the dense-string FP64 sigma, on the largest request
+(MFMA-bound; FP32 beside it), timed with HIP events on the library's stream; the pair-list
+sigma's HBM roofline on the full-CI 4a4b/12 space is reported beside it.  The CPU baseline
 is the reference's compute_hv (oracle/_ref) timed for one sigma pass over a sample
 instance and converted to solves/s through the workload's total sigma work.
 """
@@ -35,7 +36,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM = 8.0e12        # MI355X HBM3E, bytes/s
 METRIC = "Davidson solves/s + p99 latency at 1e-8 Ha, mixed vs plain FP64, 1 GPU"
 METHOD_NAMES = {1: "pairs64", 2: "pairs32", 3: "pairs_sym64", 4: "direct64", 5: "direct32",
-                6: "fast64", 7: "fast32", 8: "fci64", 9: "fci32"}
+                6: "fast64", 7: "fast32", 8: "fci64", 9: "fci32", 10: "gemm64", 11: "gemm32"}
 
 
 def log(msg):
@@ -53,10 +54,10 @@ def percentile(xs, q):
 
 
 def solve_kw(J, spec, N, restol):
-    nr = spec["nroots"]
-    return dict(nroots=nr, restol=restol, krymin=min(nr + 1, N),
-                krymax=min(max(3 * nr + 1, 12), N), refdim=min(64, N), guess=J.GUESS_DIAG,
-                totfrze=J.SYNTH_ENUC, vectors=False)
+    from jayci_amd.serve import solve_params
+    kw = solve_params(spec, N, restol)
+    kw["vectors"] = False
+    return kw
 
 
 def make_instance(J, ctx, spec, seed, flags):
@@ -146,6 +147,34 @@ def roofline_of(J, ctx, spec, seed):
     return out
 
 
+PEAK_MFMA = {64: 78.6, 32: 157.3}   # MI355X dense MFMA peaks, TFLOP/s
+
+
+def roofline_dense(J, ctx, spec, seed):
+    """Dense-string sigma (k_gE + k_gss + k_gga, one vector) on one full-CI instance, FP64
+    and FP32: algorithmic FLOPs 2*m1^2*N (opposite-spin product with (P|Q)) +
+    2*(Na^2*Nb + Na*Nb^2) (same-spin products) over the HIP-event time of one sigma."""
+    from math import comb
+    sp, h = make_instance(J, ctx, spec, seed, J.DIRECT_TABLES)
+    n = spec["norbs"]
+    na, nb = comb(n, spec["na"]), comb(n, spec["nb"])
+    N = na * nb
+    m1 = n * (n + 1) // 2
+    flops = 2.0 * m1 * m1 * N + 2.0 * (na * na * nb + na * nb * nb)
+    out = {}
+    for prec, meth in ((64, J.SIGMA_GEMM64), (32, J.SIGMA_GEMM32)):
+        ms = h.bench_sigma(meth, 1, 20)
+        ach = flops / (ms * 1e-3) / 1e12
+        out[prec] = {"bound": "mfma", "achieved": ach, "peak": PEAK_MFMA[prec],
+                     "unit": "TFLOP/s", "frac": ach / PEAK_MFMA[prec], "traffic": None,
+                     "kernel": "dense-string sigma %s (k_gE + k_gss + k_gga), N=%d (%da%db/%d)"
+                               % ("FP64" if prec == 64 else "FP32", N, spec["na"], spec["nb"], n),
+                     "ms_per_launch": ms, "algorithmic_flops": flops}
+    h.close()
+    sp.close()
+    return out
+
+
 def run_plain(J, ctx, stream, restol, precision):
     """One request at a time in arrival order (no batching).  Returns (results, wall)."""
     out = []
@@ -217,6 +246,7 @@ def bench_stream(J, ctx, args, rate, seed):
     lat_b = [r.t_done - r.arrival for r in done_b]
     lat_p = [lat for _, lat, _, _ in plain]
     elements = sum(sum(fci_pairs(stream[i][1])) * st["nsigma64"] for i, _, _, st in plain)
+    big = max(range(len(stream)), key=lambda i: ndets[i])
     # CPU sample: the largest request of at most 250,000 determinants
     small = [i for i in range(len(stream)) if ndets[i] <= 250000] or [0]
     samp = max(small, key=lambda i: ndets[i])
@@ -242,6 +272,7 @@ def bench_stream(J, ctx, args, rate, seed):
                         "sigma_s": sig_b, "path": "served: continuous batching, FP64"},
         "mixed_over_plain": wall_p / wall_m,
         "_cpu": (stream[samp][1], stream[samp][2], (len(plain), elements)),
+        "_big": (stream[big][1], stream[big][2]),
     }
 
 
@@ -293,6 +324,7 @@ def bench_fci12(J, ctx, args):
                        "nsigma64": spl["nsigma64"]},
         "mixed_over_plain": sum(lp) / sum(lm),
         "_cpu": (spec, wl["seed"], (1, sum(fci_pairs(spec)) * spl["nsigma64"])),
+        "_big": (spec, wl["seed"]),
     }
 
 
@@ -322,11 +354,16 @@ def main():
     else:
         res = bench_stream(J, ctx, args, 0.0, 300)
     sample, sseed, work = res.pop("_cpu")
+    bspec, bseed = res.pop("_big")
     from jayci_amd.workloads import WORKLOADS
     wl = WORKLOADS["fci12"]
-    roof = roofline_of(J, ctx, {k: wl[k] for k in ("na", "nb", "norbs", "ndocc", "nactv",
-                                                  "xlvl", "rule")}, wl["seed"])
-    log("roofline %.1f GB/s (%.3f of peak)" % (roof["achieved"], roof["frac"]))
+    plain_roof = roofline_of(J, ctx, {k: wl[k] for k in ("na", "nb", "norbs", "ndocc", "nactv",
+                                                        "xlvl", "rule")}, wl["seed"])
+    log("pair-list sigma %.1f GB/s (%.3f of HBM peak)" % (plain_roof["achieved"], plain_roof["frac"]))
+    dense = roofline_dense(J, ctx, bspec, bseed)
+    roof = dense[64]
+    log("dense-string sigma FP64 %.1f TFLOP/s (%.3f), FP32 %.1f TFLOP/s (%.3f)"
+        % (dense[64]["achieved"], dense[64]["frac"], dense[32]["achieved"], dense[32]["frac"]))
     cpu = None
     if not args.no_cpu:
         try:
@@ -343,6 +380,7 @@ def main():
     if "served_fp64" in res:
         line["served_fp64"] = res["served_fp64"]
     line.update({"mixed_over_plain": res["mixed_over_plain"], "roofline": roof,
+                 "roofline_fp32": dense[32], "roofline_pair_list": plain_roof,
                  "cpu_baseline": cpu})
     print(json.dumps(line))
 

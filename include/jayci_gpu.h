@@ -151,6 +151,9 @@ typedef struct jci_ham jci_ham;
 #define JCI_SIGMA_FAST32 7  /* matrix-free, excitation tables, FP32 accumulation */
 #define JCI_SIGMA_FCI64 8   /* matrix-free, single-block (FCI) spaces, n <= 16, FP64 */
 #define JCI_SIGMA_FCI32 9   /* ... FP32 accumulation */
+#define JCI_SIGMA_GEMM64 10 /* dense-string full-CI sigma on MFMA (n <= 15, <= 65535 strings
+                               per spin), FP64 */
+#define JCI_SIGMA_GEMM32 11 /* ... FP32 operands and accumulation */
 
 #define JCI_PREC_FP64 0
 #define JCI_PREC_MIXED 1
@@ -209,6 +212,8 @@ typedef struct {
     int guess;            /* JCI_GUESS_* */
     double totfrze;       /* added to the eigenvalues (nuclear repulsion + frozen core) */
     int verbose;
+    int nbuffer;          /* extra Ritz pairs expanded every iteration but not required to
+                             converge (guards against converging onto a higher root) */
 } jci_solve_params;
 
 typedef struct {
@@ -240,11 +245,11 @@ int jci_bench_copy(jci_ctx *ctx, long long bytes, int reps, double *ms_per_launc
 
 /* Autotuner: times every sigma variant the Hamiltonian holds (pair list FP64/FP32;
  * matrix-free FP64/FP32 with beta tiles of 64, 128 and 256 strings; the single-block
- * kernel where it applies) on nvec vectors and
+ * kernel and the dense-string MFMA sigma where they apply) on nvec vectors and
  * makes the fastest FP64 / FP32 ones what JCI_SIGMA_AUTO resolves to.  times (may be
  * NULL) gets ms per launch indexed by JCI_SIGMA_* (0 where not available). */
 int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32, double *times);
-/* times[] above has JCI_SIGMA_FCI32 + 1 entries */
+/* times[] above has JCI_SIGMA_GEMM32 + 1 entries */
 /* beta strings per tile of the matrix-free sigma (0 without excitation tables) */
 int jci_ham_fast_width(const jci_ham *h);
 
@@ -262,6 +267,10 @@ typedef struct jci_batch jci_batch;
 int jci_batch_create(jci_ctx *ctx, int capacity, int max_norbs, jci_batch **out);
 void jci_batch_destroy(jci_batch *b);
 int jci_batch_set_graphs(jci_batch *b, int on);
+/* device pool for the opposite-spin intermediates of the dense-string sigma (default 24 GiB,
+ * allocated at the first such sigma, grown to the largest single vector when smaller);
+ * a fuller step runs its sigma in several rounds */
+int jci_batch_set_pool(jci_batch *b, long long bytes);
 /* device bytes a slot of ndets determinants takes (basis, sigma and staging vectors) */
 long long jci_batch_bytes(long long ndets, int krymax);
 int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, int *slot);

@@ -18,7 +18,8 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "jayci_gpu.h")
 RULE_SERIAL, RULE_MRCISD = 0, 1
 PAIRS_FP64, PAIRS_FP32, PAIRS_UPPER, DIRECT_TABLES = 1, 2, 4, 8
 (SIGMA_AUTO, SIGMA_PAIRS64, SIGMA_PAIRS32, SIGMA_PAIRS_SYM64, SIGMA_DIRECT64, SIGMA_DIRECT32,
- SIGMA_FAST64, SIGMA_FAST32, SIGMA_FCI64, SIGMA_FCI32) = range(10)
+ SIGMA_FAST64, SIGMA_FAST32, SIGMA_FCI64, SIGMA_FCI32, SIGMA_GEMM64, SIGMA_GEMM32) = range(12)
+NMETHODS = 12
 PREC_FP64, PREC_MIXED = 0, 1
 GUESS_BLOCK, GUESS_USER, GUESS_DIAG = 1, 2, 3
 MAXROOTS = 8
@@ -35,7 +36,8 @@ _i, _ll, _d = ctypes.c_int, ctypes.c_longlong, ctypes.c_double
 class SolveParams(ctypes.Structure):
     _fields_ = [("maxiter", _i), ("krymin", _i), ("krymax", _i), ("nroots", _i), ("refdim", _i),
                 ("restol", _d), ("precision", _i), ("mixed_switch", _d), ("sigma_fp64", _i),
-                ("sigma_fp32", _i), ("guess", _i), ("totfrze", _d), ("verbose", _i)]
+                ("sigma_fp32", _i), ("guess", _i), ("totfrze", _d), ("verbose", _i),
+                ("nbuffer", _i)]
 
 
 class SolveStats(ctypes.Structure):
@@ -87,6 +89,7 @@ _sig("jci_ham_fast_width", _i, [_P])
 _sig("jci_batch_create", _i, [_P, _i, _i, ctypes.POINTER(_P)])
 _sig("jci_batch_destroy", None, [_P])
 _sig("jci_batch_set_graphs", _i, [_P, _i])
+_sig("jci_batch_set_pool", _i, [_P, _ll])
 _sig("jci_batch_bytes", _ll, [_ll, _i])
 _sig("jci_batch_add", _i, [_P, _P, ctypes.POINTER(SolveParams), ctypes.POINTER(_i)])
 _sig("jci_batch_step", _i, [_P, ctypes.POINTER(_i)])
@@ -325,10 +328,10 @@ class Hamiltonian:
     def autotune(self, nvec=1, reps=5):
         """Time all sigma variants; returns (best64, best32, {method: ms}, tile width)."""
         b64, b32 = _i(), _i()
-        t = np.zeros(10)
+        t = np.zeros(NMETHODS)
         _check(_L.jci_ham_autotune(self._h, nvec, reps, ctypes.byref(b64), ctypes.byref(b32), _p(t)),
                "jci_ham_autotune")
-        return b64.value, b32.value, {m: t[m] for m in range(10) if t[m] > 0}, \
+        return b64.value, b32.value, {m: t[m] for m in range(NMETHODS) if t[m] > 0}, \
             _L.jci_ham_fast_width(self._h)
 
     def sigma(self, x, method=SIGMA_AUTO):
@@ -347,14 +350,15 @@ class Hamiltonian:
 
     def params(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
                precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
-               sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False):
+               sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0):
         return make_params(self.space.ndets, nroots, restol, maxiter, krymin, krymax, refdim,
-                           precision, mixed_switch, guess, totfrze, sigma_fp64, sigma_fp32, verbose)
+                           precision, mixed_switch, guess, totfrze, sigma_fp64, sigma_fp32, verbose,
+                           nbuffer)
 
     def solve(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
               precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, guess_vectors=None,
               totfrze=0.0, sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, vectors=True,
-              verbose=False, check=True):
+              verbose=False, check=True, nbuffer=0):
         p = SolveParams()
         _L.jci_solve_defaults(ctypes.byref(p))
         p.nroots = nroots
@@ -370,6 +374,7 @@ class Hamiltonian:
         p.sigma_fp64 = sigma_fp64
         p.sigma_fp32 = sigma_fp32
         p.verbose = 1 if verbose else 0
+        p.nbuffer = nbuffer
         N = self.space.ndets
         ev = np.zeros(nroots)
         vec = np.zeros((nroots, N)) if vectors else None
@@ -389,7 +394,7 @@ class Hamiltonian:
 
 def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
                 precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
-                sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False):
+                sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0):
     p = SolveParams()
     _L.jci_solve_defaults(ctypes.byref(p))
     p.nroots = nroots
@@ -405,13 +410,14 @@ def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=N
     p.sigma_fp64 = sigma_fp64
     p.sigma_fp32 = sigma_fp32
     p.verbose = 1 if verbose else 0
+    p.nbuffer = nbuffer
     return p
 
 
 class Batch:
     """Lockstep batched Davidson over many instances (continuous batching)."""
 
-    def __init__(self, ctx, capacity, max_norbs=64, graphs=True):
+    def __init__(self, ctx, capacity, max_norbs=64, graphs=True, pool_bytes=None):
         self.ctx = ctx
         h = _P()
         _check(_L.jci_batch_create(ctx._h, capacity, max_norbs, ctypes.byref(h)), "jci_batch_create")
@@ -419,6 +425,8 @@ class Batch:
         self._keep = {}
         if not graphs:
             _L.jci_batch_set_graphs(h, 0)
+        if pool_bytes is not None:
+            _check(_L.jci_batch_set_pool(h, int(pool_bytes)), "jci_batch_set_pool")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjayci_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["kernels.hip", "sigma_fast.hip", "batch_kernels.hip"]
+HIP_SOURCES = ["kernels.hip", "sigma_fast.hip", "sigma_gemm.hip", "batch_kernels.hip"]
 CXX_SOURCES = ["space.cpp", "jci.cpp", "capi.cpp", "batch.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden", "-Wno-unused-result", "-Wno-unused-value", "-I" + CSRC,
@@ -45,7 +45,7 @@ def build(verbose=False):
         cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip" if src.endswith(".hip") else "c++",
                                            "-c", path, "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
+            cmd = [HIPCC] + COMMON + DEVICE + ["-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
         out = _run(cmd)
         if verbose and out.strip():
             print(out)

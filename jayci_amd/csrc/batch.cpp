@@ -7,7 +7,10 @@
 // orthonormalisation, and a batched append.  Instances may be added between steps (the
 // serving loop admits requests this way) and leave when converged.  Each instance keeps
 // its own precision phase: FP32 sigma until its residual norm falls below mixed_switch,
-// then its whole basis is re-multiplied in FP64.
+// then its basis collapses onto the current Ritz vectors, which are re-multiplied in FP64.
+// Full-CI instances take the dense-string MFMA sigma (grouped kernels over all of them, the
+// opposite-spin intermediates in a shared pool, several rounds when the pool is full); the
+// others take the persistent table kernel.
 #include "batch.h"
 #include "slater.h"
 
@@ -39,10 +42,12 @@ struct BInst {
     jci_ham *h = nullptr;
     jci_solve_params p;
     FastArgs fa;
+    int gf = 0;                        // dense-string sigma
+    GemmFci g;
     long long N = 0, ld = 0;
     double *V = nullptr, *S = nullptr, *T = nullptr, *X = nullptr, *Y = nullptr;
     float *X32 = nullptr;
-    int m = 0, krymin = 0, krymax = 0, nroots = 0;
+    int m = 0, krymin = 0, krymax = 0, nroots = 0, ntrack = 0;
     int phase32 = 0;
     int sig_first = 0, sig_count = 0;  // basis columns whose sigma is pending
     int iters = 0;
@@ -78,6 +83,11 @@ struct jci_batch {
     size_t partial_cap;
     std::vector<GraphKey> graphs;
     int use_graphs;
+    // dense-string sigma: launch descriptors and the pool of opposite-spin intermediates
+    GfHeader *dGH, *hGH;
+    GfTask *dGT, *hGT;
+    void *epool;
+    size_t epool_bytes, epool_want;
     long long steps;
     double sigma_seconds;
 };
@@ -115,6 +125,13 @@ JCI_API int jci_batch_create(jci_ctx *c, int capacity, int max_norbs, jci_batch 
     JCI_CHECK(hipHostMalloc(&b->hout, sizeof(double) * b->out_cap, 0));
     b->partial_cap = (size_t)capacity * 64 * JCI_MAXBASIS * JCI_MAXBASIS;
     JCI_CHECK(hipMalloc(&b->dpartial, sizeof(double) * b->partial_cap));
+    JCI_CHECK(hipMalloc(&b->dGH, sizeof(GfHeader)));
+    JCI_CHECK(hipHostMalloc(&b->hGH, sizeof(GfHeader), 0));
+    JCI_CHECK(hipMalloc(&b->dGT, sizeof(GfTask) * capacity));
+    JCI_CHECK(hipHostMalloc(&b->hGT, sizeof(GfTask) * capacity, 0));
+    b->epool = nullptr;
+    b->epool_bytes = 0;
+    b->epool_want = (size_t)24 << 30;
     *out = b;
     return 0;
 }
@@ -139,10 +156,24 @@ JCI_API void jci_batch_destroy(jci_batch *b)
     hipFree(b->dcoef); hipHostFree(b->hcoef);
     hipFree(b->dout); hipHostFree(b->hout);
     hipFree(b->dpartial);
+    hipFree(b->dGH); hipHostFree(b->hGH);
+    hipFree(b->dGT); hipHostFree(b->hGT);
+    hipFree(b->epool);
     delete b;
 }
 
 JCI_API int jci_batch_set_graphs(jci_batch *b, int on) { b->use_graphs = on; return 0; }
+
+JCI_API int jci_batch_set_pool(jci_batch *b, long long bytes)
+{
+    if (bytes < (1LL << 20)) { jci_set_error("batch: pool below 1 MiB"); return -1; }
+    hipStreamSynchronize(b->c->stream);
+    hipFree(b->epool);
+    b->epool = nullptr;
+    b->epool_bytes = 0;
+    b->epool_want = (size_t)bytes;
+    return 0;
+}
 
 // bytes of device memory an instance of N determinants takes in the batch
 JCI_API long long jci_batch_bytes(long long N, int krymax)
@@ -165,20 +196,28 @@ JCI_API int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, i
     s.h = h;
     s.p = *p;
     s.fa = fa;
+    s.gf = jci_ham_gemmfci(h, &s.g) == 0;
     s.N = jci_ham_ndets(h);
     s.ld = (s.N + 63) / 64 * 64;
     s.nroots = p->nroots;
     s.krymax = (int)std::min<long long>(p->krymax, s.N);
-    s.krymin = std::min(std::max(p->krymin, p->nroots), s.krymax);
     if (s.nroots < 1 || s.nroots > JCI_MAXROOTS || s.nroots > s.N || s.krymax > JCI_MAXBASIS ||
-        s.krymin < s.nroots) {
+        p->nbuffer < 0) {
+        jci_set_error("batch: bad root / subspace sizes");
+        return -1;
+    }
+    s.ntrack = (int)std::min(std::min<long long>(s.nroots + p->nbuffer, JCI_MAXROOTS),
+                             std::min<long long>(s.N, std::max(s.nroots, s.krymax / 2)));
+    s.krymin = std::min(std::max(p->krymin, s.ntrack), s.krymax);
+    if (s.krymin < s.nroots) {
         jci_set_error("batch: bad root / subspace sizes");
         return -1;
     }
     size_t vb = sizeof(double) * (size_t)s.ld;
     if (hipMalloc(&s.V, vb * s.krymax) != hipSuccess || hipMalloc(&s.S, vb * s.krymax) != hipSuccess ||
-        hipMalloc(&s.T, vb * JCI_MAXK) != hipSuccess || hipMalloc(&s.X, vb * JCI_MAXK) != hipSuccess ||
-        hipMalloc(&s.Y, vb * JCI_MAXK) != hipSuccess ||
+        hipMalloc(&s.T, vb * JCI_MAXK) != hipSuccess ||
+        (!s.gf && (hipMalloc(&s.X, vb * JCI_MAXK) != hipSuccess ||
+                   hipMalloc(&s.Y, vb * JCI_MAXK) != hipSuccess)) ||
         hipMalloc(&s.X32, sizeof(float) * s.ld * JCI_MAXK) != hipSuccess) {
         free_inst(s);
         jci_set_error("batch: out of device memory for an instance of %lld determinants", s.N);
@@ -287,14 +326,137 @@ static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vec
     return 0;
 }
 
+static hipGraphExec_t find_graph(jci_batch *b, int prec, int K);
+static int run_sigma_tab(jci_batch *b, const std::vector<int> &ids, int prec);
+
+// dense-string sigma of every pending column of the listed instances, in rounds that fit the
+// intermediate pool
+static int run_sigma_gf(jci_batch *b, const std::vector<int> &ids, int prec)
+{
+    if (ids.empty()) return 0;
+    jci_ctx *c = b->c;
+    const size_t esz = prec == 64 ? 8 : 4;
+    size_t need = 0;
+    for (int i : ids) need = std::max(need, (size_t)b->in[i].g.m1 * (size_t)b->in[i].N * esz);
+    if (!b->epool || b->epool_bytes < need) {
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+        hipFree(b->epool);
+        b->epool = nullptr;
+        b->epool_bytes = 0;
+        size_t want = std::max(need, b->epool_want);
+        if (hipMalloc(&b->epool, want) != hipSuccess) {
+            jci_set_error("batch: out of device memory for the sigma pool (%zu bytes)", want);
+            return -3;
+        }
+        b->epool_bytes = want;
+    }
+    for (;;) {
+        JCI_CHECK(hipStreamSynchronize(c->stream));   // pinned descriptors free again
+        int n = 0;
+        size_t used = 0;
+        long long base[4] = {0, 0, 0, 0};
+        std::vector<int> who, kk;
+        for (int i : ids) {
+            BInst &s = b->in[i];
+            if (s.sig_count <= 0) continue;
+            const size_t per = (size_t)s.g.m1 * s.N * esz;
+            const size_t room = (b->epool_bytes - used) / per;
+            const int k = (int)std::min<size_t>(std::min<size_t>(s.sig_count, JCI_MAXK), room);
+            if (k < 1) continue;
+            GfTask &t = b->hGT[n];
+            memset(&t, 0, sizeof(t));
+            t.g = s.g;
+            t.K = k;
+            t.x64 = s.V + (size_t)s.sig_first * s.ld;
+            t.ldx64 = s.ld;
+            t.x32 = prec == 32 ? s.X32 : nullptr;
+            t.y = s.S + (size_t)s.sig_first * s.ld;
+            t.ldy = s.ld;
+            t.E = (char *)b->epool + used;
+            long long u[4];
+            jk_gf_units(prec, t, u);
+            for (int q = 0; q < 4; q++) { t.base[q] = base[q]; base[q] += u[q]; }
+            used += (per * k + 255) / 256 * 256;
+            who.push_back(i);
+            kk.push_back(k);
+            n++;
+        }
+        if (n == 0) break;
+        b->hGH->n = n;
+        for (int q = 0; q < 4; q++) b->hGH->total[q] = base[q];
+        JCI_CHECK(hipMemcpyAsync(b->dGH, b->hGH, sizeof(GfHeader), hipMemcpyHostToDevice, c->stream));
+        JCI_CHECK(hipMemcpyAsync(b->dGT, b->hGT, sizeof(GfTask) * n, hipMemcpyHostToDevice, c->stream));
+        double t0 = tnow();
+        if (b->use_graphs) {
+            hipGraphExec_t ex = find_graph(b, prec, -1);
+            if (!ex) {
+                hipGraph_t g;
+                JCI_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int rc = jk_gf_group(prec, b->dGH, b->dGT, nullptr, c->stream);
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) { jci_set_error("batch: graph capture failed: %s", hipGetErrorString(e)); return -2; }
+                JCI_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                hipGraphDestroy(g);
+                b->graphs.push_back(GraphKey{prec, -1, ex});
+            }
+            JCI_CHECK(hipGraphLaunch(ex, c->stream));
+        } else {
+            RCHECK(jk_gf_group(prec, b->dGH, b->dGT, b->hGH, c->stream));
+        }
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+        b->sigma_seconds += tnow() - t0;
+        for (size_t t = 0; t < who.size(); t++) {
+            BInst &s = b->in[who[t]];
+            s.sig_first += kk[t];
+            s.sig_count -= kk[t];
+            if (prec == 64) s.ns64 += kk[t]; else s.ns32 += kk[t];
+        }
+    }
+    return 0;
+}
+
+// sigma of every pending column of the listed instances of one precision
+static int run_sigma(jci_batch *b, const std::vector<int> &ids, int prec)
+{
+    std::vector<int> gfi, tab;
+    for (int i : ids) (b->in[i].gf ? gfi : tab).push_back(i);
+    RCHECK(run_sigma_gf(b, gfi, prec));
+    // the table kernel takes at most JCI_MAXK columns per instance and launch
+    for (;;) {
+        std::vector<int> now;
+        std::vector<int> rest;
+        for (int i : tab) {
+            BInst &s = b->in[i];
+            if (s.sig_count <= 0) continue;
+            now.push_back(i);
+            rest.push_back(s.sig_count - std::min(s.sig_count, JCI_MAXK));
+            s.sig_count = std::min(s.sig_count, JCI_MAXK);
+        }
+        if (now.empty()) break;
+        std::vector<int> first(now.size());
+        for (size_t t = 0; t < now.size(); t++) {
+            first[t] = b->in[now[t]].sig_first + b->in[now[t]].sig_count;
+        }
+        RCHECK(run_sigma_tab(b, now, prec));
+        for (size_t t = 0; t < now.size(); t++) {
+            BInst &s = b->in[now[t]];
+            s.sig_first = first[t];
+            s.sig_count = rest[t];
+        }
+    }
+    return 0;
+}
+
 static hipGraphExec_t find_graph(jci_batch *b, int prec, int K)
 {
     for (auto &g : b->graphs) if (g.prec == prec && g.K == K) return g.exec;
     return nullptr;
 }
 
-// sigma for the pending columns of every listed instance of one precision
-static int run_sigma(jci_batch *b, const std::vector<int> &ids, int prec)
+// table-kernel sigma for up to JCI_MAXK pending columns of every listed instance (the
+// caller has cut sig_count to at most JCI_MAXK)
+static int run_sigma_tab(jci_batch *b, const std::vector<int> &ids, int prec)
 {
     int n = (int)ids.size();
     if (n == 0) return 0;
@@ -434,7 +596,7 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
         long long blk = 0, pofs = 0;
         for (size_t t = 0; t < act.size(); t++) {
             BInst &s = b->in[act[t]];
-            int m = s.m, k = s.nroots;
+            int m = s.m, k = std::min(s.ntrack, s.m);
             BDesc &d = b->hD[t];
             memset(&d, 0, sizeof(d));
             for (int i = 0; i < m; i++)
@@ -456,38 +618,65 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
     }
     // 4. decisions per instance
     std::vector<int> grow;          // instances adding vectors this step
-    std::vector<int> recompute;     // precision hand-over
+    std::vector<int> handover;      // FP32 -> FP64: collapse onto Ritz vectors, re-multiply
     for (size_t t = 0; t < act.size(); t++) {
         BInst &s = b->in[act[t]];
+        const int nt = std::min(s.ntrack, s.m);
+        double rall[JCI_MAXK];
+        for (int r = 0; r < nt; r++) rall[r] = sqrt(std::max(0.0, b->hout[ofs[t] + r]));
         double rmax = 0.0;
         for (int r = 0; r < s.nroots; r++) {
-            s.rn[r] = sqrt(std::max(0.0, b->hout[ofs[t] + r]));
+            s.rn[r] = rall[r];
             rmax = std::max(rmax, s.rn[r]);
         }
         for (int r = 0; r < s.nroots; r++) s.ev[r] = s.theta[r] + s.p.totfrze;
         if (s.phase32 && (rmax < s.p.mixed_switch || s.iters >= s.p.maxiter)) {
             s.phase32 = 0;
-            s.sig_first = 0;
-            s.sig_count = s.m;
-            recompute.push_back(act[t]);
+            handover.push_back(act[t]);
             continue;
         }
-        s.todo.clear();
+        bool req_done = true;
         for (int r = 0; r < s.nroots; r++)
-            if (!(s.rn[r] < s.p.restol)) s.todo.push_back(r);
-        if (s.todo.empty()) { s.done = 1; s.t_done = tnow(); continue; }
+            if (!(s.rn[r] < s.p.restol)) req_done = false;
+        if (req_done) { s.done = 1; s.t_done = tnow(); continue; }
         if (s.iters >= s.p.maxiter) { s.done = 1; s.failed = 1; s.t_done = tnow(); continue; }
+        s.todo.clear();
+        for (int r = 0; r < nt; r++)
+            if (!(rall[r] < s.p.restol)) s.todo.push_back(r);
+        // a full subspace collapses onto kc Ritz vectors; buffer corrections that do not fit
+        // next to them are dropped
+        if (s.m + (int)s.todo.size() > s.krymax) {
+            int kc = std::max(s.krymin, nt);
+            kc = std::min(kc, s.krymax - (int)s.todo.size());
+            if (kc < s.nroots) kc = s.nroots;
+            if (kc + (int)s.todo.size() > s.krymax) s.todo.resize(s.krymax - kc);
+        }
         grow.push_back(act[t]);
     }
-    // the FP64 re-multiplication of a whole basis may exceed JCI_MAXK vectors: chunk it
-    for (int i : recompute) {
-        BInst &s = b->in[i];
-        int total = s.m;
-        for (int f = 0; f < total; f += JCI_MAXK) {
-            s.sig_first = f;
-            s.sig_count = std::min(JCI_MAXK, total - f);
-            std::vector<int> one{i};
-            RCHECK(run_sigma(b, one, 64));
+    // hand-over: V <- V Y[:, :kc] (the lowest kc Ritz vectors of the FP32 phase); their FP64
+    // sigma is pending for the next step
+    if (!handover.empty()) {
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+        std::vector<int> ids;
+        std::vector<double *> A, O;
+        std::vector<int> m, k;
+        std::vector<size_t> cofs;
+        size_t co = 0;
+        for (int i : handover) {
+            BInst &s = b->in[i];
+            int kc = std::min(s.m, s.krymin);
+            for (int r = 0; r < s.m; r++)
+                for (int j = 0; j < kc; j++) b->hcoef[co + (size_t)r * kc + j] = s.Yf[(size_t)j * s.m + r];
+            ids.push_back(i); A.push_back(s.V); O.push_back(s.V); m.push_back(s.m); k.push_back(kc);
+            cofs.push_back(co);
+            co += (size_t)s.m * kc;
+        }
+        RCHECK(run_combine(b, ids, A, O, m, k, cofs, 0.0, co));
+        for (size_t t = 0; t < ids.size(); t++) {
+            BInst &s = b->in[ids[t]];
+            s.m = k[t];
+            s.sig_first = 0;
+            s.sig_count = s.m;
         }
     }
     if (grow.empty()) return 0;
@@ -502,12 +691,15 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
         for (int i : grow) {
             BInst &s = b->in[i];
             int nk = (int)s.todo.size();
-            if (nk == s.nroots) continue;
-            for (int r = 0; r < s.nroots; r++)
+            const int nt = std::min(s.ntrack, s.m);
+            bool ident = nk == nt;
+            for (int j = 0; j < nk && ident; j++) ident = s.todo[j] == j;
+            if (ident) continue;
+            for (int r = 0; r < nt; r++)
                 for (int j = 0; j < nk; j++) b->hcoef[co + (size_t)r * nk + j] = (s.todo[j] == r) ? 1.0 : 0.0;
-            ids.push_back(i); A.push_back(s.T); O.push_back(s.T); m.push_back(s.nroots); k.push_back(nk);
+            ids.push_back(i); A.push_back(s.T); O.push_back(s.T); m.push_back(nt); k.push_back(nk);
             cofs.push_back(co);
-            co += (size_t)s.nroots * nk;
+            co += (size_t)nt * nk;
         }
         RCHECK(run_combine(b, ids, A, O, m, k, cofs, 0.0, co));
     }
@@ -523,7 +715,7 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             BInst &s = b->in[i];
             int nk = (int)s.todo.size();
             if (s.m + nk <= s.krymax) continue;
-            int kc = std::max(s.krymin, s.nroots);
+            int kc = std::max(s.krymin, std::min(s.ntrack, s.m));
             kc = std::min(kc, s.krymax - nk);
             if (kc < s.nroots) kc = s.nroots;
             for (int r = 0; r < s.m; r++)

@@ -2,6 +2,7 @@
 #pragma once
 #include "jci_internal.h"
 #include "sigma_fast.h"
+#include "sigma_gemm.h"
 
 // launch header in device memory: the persistent kernels read their work counts here
 struct BHeader {
@@ -54,3 +55,4 @@ const double *jci_ham_hd(jci_ham *h);
 long long jci_ham_ndets(jci_ham *h);
 jci_ctx *jci_ham_ctx(jci_ham *h);
 int jci_ham_guess_block(jci_ham *h, int refdim, int nguess, int diag_guess, double *V, long long ld);
+int jci_ham_gemmfci(jci_ham *h, GemmFci *out);

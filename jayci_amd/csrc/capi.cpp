@@ -525,7 +525,7 @@ JCI_API int dvdalg(struct det *dlist, int ndets, double *moints1, double *moints
         JCI_CHECK(hipMemcpyAsync(d_pre, hdgls, sizeof(double) * ndets, hipMemcpyHostToDevice,
                                  op->c->stream));
     }
-    SolveOp so;
+    SolveOp so{};
     so.self = op;
     so.ctx = op->c;
     so.N = op->n;

@@ -3,6 +3,7 @@
 #include "jci_internal.h"
 #include "slater.h"
 #include "sigma_fast.h"
+#include "sigma_gemm.h"
 #include "../../include/jayci_gpu.h"
 
 #include <math.h>
@@ -263,6 +264,25 @@ struct jci_ham {
     int have_fci;               // single-block space with n <= 16: FCI kernel usable
     FciArgs fci;
     int tuned64, tuned32;       // autotuned default methods (0 = not tuned)
+    // dense-string MFMA sigma (full-CI spaces)
+    int have_gf;
+    GemmFci gf;
+    double *d_gfH64[2];
+    float *d_gfH32[2];
+    double *d_gfM64;
+    float *d_gfM32;
+    uint32_t *d_gfla, *d_gflb;
+    void *d_gfE;                // opposite-spin intermediate E
+    size_t gfE_bytes;
+    float *d_gfX32;             // FP32 copies of the input vectors, vector-major
+    size_t gfX32_len;
+    double *d_gfT;              // vector-major staging for determinant-major callers
+    size_t gfT_len;
+    // launch descriptors of the single-instance path: one slot per vector chunk, host copies
+    // pinned; a slot is uploaded only when its content changes
+    GfTask *h_gft, *d_gft;
+    GfHeader *h_gfh, *d_gfh;
+    int gf_valid[JCI_MAXK];
 };
 
 static size_t m1_len(int n) { return (size_t)n * (n + 1) / 2; }
@@ -304,6 +324,67 @@ static int build_pairs(jci_ham *h, int flags)
     RCHECK(jk_row_fill(s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom, h->d_m1, h->d_m2,
                        h->d_rowptr, h->d_col, h->d_val64, h->d_val32, h->upper, c->stream));
     JCI_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// dense same-spin matrices, replacement lists and padded (P|Q) of the dense-string sigma;
+// taken when the space is one alpha group times one beta group (full CI), n <= 15 and the
+// dense matrices stay below 8 GB
+static int build_gf(jci_ham *h)
+{
+    jci_space *s = h->sp;
+    jci_ctx *c = s->ctx;
+    const SpaceDesc &D = s->h;
+    if (D.nblk != 1 || D.s[0].ngrp != 1 || D.s[1].ngrp != 1) return 0;
+    if (D.s[0].gstart[0] != 0 || D.s[1].gstart[0] != 0) return 0;
+    const int n = D.norbs, m1 = n * (n + 1) / 2, MP = (m1 + 15) / 16 * 16;
+    const long long na = D.s[0].gn[0], nb = D.s[1].gn[0];
+    if (MP > GF_MAXMP || na > GF_MAXSTR || nb > GF_MAXSTR) return 0;
+    if (D.allmask != (n >= 64 ? ~0ULL : ((1ULL << n) - 1ULL))) return 0;
+    const bool same = s->na == s->nb;
+    if ((double)(na * na + (same ? 0 : nb * nb)) * 12.0 > 8e9) return 0;
+    const int sa = s->na * (n - s->na), sb = s->nb * (n - s->nb);
+    if (h->nexc1[0] != na * sa || h->nexc1[1] != nb * sb) return 0;
+    const int La = sa + s->na, Lb = sb + s->nb;
+    JCI_CHECK(hipMalloc(&h->d_gfH64[0], sizeof(double) * na * na));
+    JCI_CHECK(hipMalloc(&h->d_gfH32[0], sizeof(float) * na * na));
+    RCHECK(jk_gf_dense(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], h->d_v1[0], h->d_off2[0],
+                       h->d_e2[0], h->d_v2[0], h->d_m1, h->d_m2, h->d_gfH64[0], c->stream));
+    RCHECK(jk_to_f32(h->d_gfH64[0], h->d_gfH32[0], na * na, c->stream));
+    if (same) {
+        h->d_gfH64[1] = h->d_gfH64[0];
+        h->d_gfH32[1] = h->d_gfH32[0];
+    } else {
+        JCI_CHECK(hipMalloc(&h->d_gfH64[1], sizeof(double) * nb * nb));
+        JCI_CHECK(hipMalloc(&h->d_gfH32[1], sizeof(float) * nb * nb));
+        RCHECK(jk_gf_dense(nb, s->d_mask[1], h->d_off1[1], h->d_e1[1], h->d_v1[1], h->d_off2[1],
+                           h->d_e2[1], h->d_v2[1], h->d_m1, h->d_m2, h->d_gfH64[1], c->stream));
+        RCHECK(jk_to_f32(h->d_gfH64[1], h->d_gfH32[1], nb * nb, c->stream));
+    }
+    JCI_CHECK(hipMalloc(&h->d_gfM64, sizeof(double) * MP * MP));
+    JCI_CHECK(hipMalloc(&h->d_gfM32, sizeof(float) * MP * MP));
+    RCHECK(jk_gf_m2pad(m1, MP, h->d_m2sq, h->d_gfM64, h->d_gfM32, c->stream));
+    JCI_CHECK(hipMalloc(&h->d_gfla, sizeof(uint32_t) * na * La));
+    JCI_CHECK(hipMalloc(&h->d_gflb, sizeof(uint32_t) * nb * Lb));
+    RCHECK(jk_gf_lists(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], La, h->d_gfla, c->stream));
+    RCHECK(jk_gf_lists(nb, s->d_mask[1], h->d_off1[1], h->d_e1[1], Lb, h->d_gflb, c->stream));
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    GemmFci &g = h->gf;
+    g.na = na;
+    g.nb = nb;
+    g.m1 = m1;
+    g.MP = MP;
+    g.La = La;
+    g.Lb = Lb;
+    g.H64[0] = h->d_gfH64[0];
+    g.H64[1] = h->d_gfH64[1];
+    g.H32[0] = h->d_gfH32[0];
+    g.H32[1] = h->d_gfH32[1];
+    g.M64 = h->d_gfM64;
+    g.M32 = h->d_gfM32;
+    g.la = h->d_gfla;
+    g.lb = h->d_gflb;
+    h->have_gf = 1;
     return 0;
 }
 
@@ -403,7 +484,7 @@ static int build_fast(jci_ham *h, int W)
         long long maxs = (long long)std::max(s->na, s->nb) * (D.norbs - std::min(s->na, s->nb));
         h->have_fci = (D.s[0].gstart[0] == 0 && D.s[1].gstart[0] == 0 && maxs <= FCI_MAXSINGLES);
     }
-    return 0;
+    return build_gf(h);
 }
 
 JCI_API int jci_ham_create(jci_space *s, const double *m1, const double *m2, int flags,
@@ -452,6 +533,13 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     }
     hipFree(h->d_m2sq);
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tile_off[wi]);
+    if (h->d_gfH64[1] != h->d_gfH64[0]) { hipFree(h->d_gfH64[1]); hipFree(h->d_gfH32[1]); }
+    hipFree(h->d_gfH64[0]); hipFree(h->d_gfH32[0]);
+    hipFree(h->d_gfM64); hipFree(h->d_gfM32); hipFree(h->d_gfla); hipFree(h->d_gflb);
+    hipFree(h->d_gfE); hipFree(h->d_gfX32); hipFree(h->d_gfT);
+    hipFree(h->d_gft); hipFree(h->d_gfh);
+    if (h->h_gft) hipHostFree(h->h_gft);
+    if (h->h_gfh) hipHostFree(h->h_gfh);
     jci_space *s = h->sp;
     free(h);
     if (--s->refs == 0 && s->closing) space_free(s);
@@ -508,10 +596,12 @@ static int resolve_method(const jci_ham *h, int method, int want32)
     if (!want32 && h->tuned64) return h->tuned64;
     if (want32) {
         if (h->d_val32) return JCI_SIGMA_PAIRS32;
+        if (h->have_gf) return JCI_SIGMA_GEMM32;
         if (h->have_fci) return JCI_SIGMA_FCI32;
         return h->have_fast ? JCI_SIGMA_FAST32 : JCI_SIGMA_DIRECT32;
     }
     if (h->d_val64) return h->upper ? JCI_SIGMA_PAIRS_SYM64 : JCI_SIGMA_PAIRS64;
+    if (h->have_gf) return JCI_SIGMA_GEMM64;
     if (h->have_fci) return JCI_SIGMA_FCI64;
     return h->have_fast ? JCI_SIGMA_FAST64 : JCI_SIGMA_DIRECT64;
 }
@@ -529,6 +619,72 @@ static int ensure_xf(jci_ham *h, long long n)
     h->xf_len = n;
     return 0;
 }
+
+static int grow(void **p, size_t *have, size_t want, const char *what)
+{
+    if (*have >= want) return 0;
+    hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    if (hipMalloc(p, want) != hipSuccess) {
+        jci_set_error("sigma: out of device memory for %s (%zu bytes)", what, want);
+        return -3;
+    }
+    *have = want;
+    return 0;
+}
+
+// dense-string sigma of K vector-major vectors: y + k*ldy = H (x + k*ldx)
+static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, double *y,
+                  long long ldy)
+{
+    jci_ctx *c = h->sp->ctx;
+    const long long N = h->sp->h.ndets;
+    const size_t esz = prec == 64 ? 8 : 4;
+    if (!h->d_gft) {
+        JCI_CHECK(hipMalloc(&h->d_gft, sizeof(GfTask) * JCI_MAXK));
+        JCI_CHECK(hipMalloc(&h->d_gfh, sizeof(GfHeader) * JCI_MAXK));
+        JCI_CHECK(hipHostMalloc(&h->h_gft, sizeof(GfTask) * JCI_MAXK, hipHostMallocDefault));
+        JCI_CHECK(hipHostMalloc(&h->h_gfh, sizeof(GfHeader) * JCI_MAXK, hipHostMallocDefault));
+        memset(h->gf_valid, 0, sizeof(h->gf_valid));
+    }
+    // vectors per pass: the intermediate E (m1 x k x N) stays within 8 GB
+    const int Kc = (int)std::max(1.0, std::min((double)K, 8e9 / ((double)h->gf.m1 * N * esz)));
+    for (int b = 0, ci = 0; b < K; b += Kc, ci++) {
+        const int k = std::min(Kc, K - b);
+        RCHECK(grow(&h->d_gfE, &h->gfE_bytes, (size_t)h->gf.m1 * k * N * esz, "the intermediate"));
+        if (prec == 32)
+            RCHECK(grow((void **)&h->d_gfX32, &h->gfX32_len, sizeof(float) * (size_t)k * N,
+                        "FP32 staging"));
+        GfTask t;
+        GfHeader hh;
+        memset(&t, 0, sizeof(t));
+        memset(&hh, 0, sizeof(hh));
+        t.g = h->gf;
+        t.K = k;
+        t.x64 = x + (size_t)b * ldx;
+        t.ldx64 = ldx;
+        t.x32 = prec == 32 ? h->d_gfX32 : nullptr;
+        t.y = y + (size_t)b * ldy;
+        t.ldy = ldy;
+        t.E = h->d_gfE;
+        hh.n = 1;
+        jk_gf_units(prec, t, hh.total);
+        if (!h->gf_valid[ci] || memcmp(&t, &h->h_gft[ci], sizeof(t)) ||
+            memcmp(&hh, &h->h_gfh[ci], sizeof(hh))) {
+            JCI_CHECK(hipStreamSynchronize(c->stream));   // earlier uploads of the slot done
+            h->h_gft[ci] = t;
+            h->h_gfh[ci] = hh;
+            JCI_CHECK(hipMemcpyAsync(h->d_gft + ci, h->h_gft + ci, sizeof(t), hipMemcpyHostToDevice, c->stream));
+            JCI_CHECK(hipMemcpyAsync(h->d_gfh + ci, h->h_gfh + ci, sizeof(hh), hipMemcpyHostToDevice, c->stream));
+            h->gf_valid[ci] = 1;
+        }
+        RCHECK(jk_gf_group(prec, h->d_gfh + ci, h->d_gft + ci, &hh, c->stream));
+    }
+    return 0;
+}
+
+static bool is_gf(int method) { return method == JCI_SIGMA_GEMM64 || method == JCI_SIGMA_GEMM32; }
 
 // y = H x on device, x and y determinant-major [N][nvec]
 static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *y)
@@ -577,6 +733,18 @@ static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *
         RCHECK(ensure_xf(h, N * nvec));
         RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
         return jk_sigma_fci(32, nvec, h->fci, h->d_xf, y, c->stream);
+    case JCI_SIGMA_GEMM64:
+    case JCI_SIGMA_GEMM32: {
+        if (!h->have_gf) { jci_set_error("sigma: dense-string sigma not applicable"); return -1; }
+        const int prec = method == JCI_SIGMA_GEMM64 ? 64 : 32;
+        if (nvec == 1) return gf_run(h, prec, 1, x, N, y, N);
+        RCHECK(grow((void **)&h->d_gfT, &h->gfT_len, sizeof(double) * 2 * (size_t)nvec * N,
+                    "vector staging"));
+        double *t1 = h->d_gfT, *t2 = h->d_gfT + (size_t)nvec * N;
+        RCHECK(jk_unpack(nvec, N, x, t1, N, c->stream));
+        RCHECK(gf_run(h, prec, nvec, t1, N, t2, N));
+        return jk_pack(nvec, N, t2, N, y, c->stream);
+    }
     default:
         jci_set_error("sigma: unknown method %d", method);
         return -1;
@@ -723,7 +891,7 @@ JCI_API int jci_bench_sigma(jci_ham *h, int method, int nvec, int reps, double *
 JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32,
                              double *times)
 {
-    double t[10] = {0};
+    double t[JCI_SIGMA_GEMM32 + 1] = {0};
     double best_w_ms = 1e300;
     int best_w = h->fast_w;
     if (h->have_fast) {
@@ -741,6 +909,10 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_FCI64, nvec, reps, &t[JCI_SIGMA_FCI64]));
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_FCI32, nvec, reps, &t[JCI_SIGMA_FCI32]));
     }
+    if (h->have_gf) {
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM64, nvec, reps, &t[JCI_SIGMA_GEMM64]));
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM32, nvec, reps, &t[JCI_SIGMA_GEMM32]));
+    }
     if (h->d_val64 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS64, nvec, reps, &t[JCI_SIGMA_PAIRS64]));
     if (h->d_val32 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS32, nvec, reps, &t[JCI_SIGMA_PAIRS32]));
     if (!h->have_fast && !h->d_val64 && !h->d_val32) {
@@ -748,15 +920,17 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_DIRECT32, nvec, 1, &t[JCI_SIGMA_DIRECT32]));
     }
     int b64 = 0, b32 = 0;
-    const int c64[] = {JCI_SIGMA_PAIRS64, JCI_SIGMA_FAST64, JCI_SIGMA_DIRECT64, JCI_SIGMA_FCI64};
-    const int c32[] = {JCI_SIGMA_PAIRS32, JCI_SIGMA_FAST32, JCI_SIGMA_DIRECT32, JCI_SIGMA_FCI32};
+    const int c64[] = {JCI_SIGMA_PAIRS64, JCI_SIGMA_FAST64, JCI_SIGMA_DIRECT64, JCI_SIGMA_FCI64,
+                       JCI_SIGMA_GEMM64};
+    const int c32[] = {JCI_SIGMA_PAIRS32, JCI_SIGMA_FAST32, JCI_SIGMA_DIRECT32, JCI_SIGMA_FCI32,
+                       JCI_SIGMA_GEMM32};
     for (int m : c64) if (t[m] > 0 && (b64 == 0 || t[m] < t[b64])) b64 = m;
     for (int m : c32) if (t[m] > 0 && (b32 == 0 || t[m] < t[b32])) b32 = m;
     h->tuned64 = b64;
     h->tuned32 = b32;
     if (best64) *best64 = b64;
     if (best32) *best32 = b32;
-    if (times) for (int m = 0; m < 10; m++) times[m] = t[m];
+    if (times) for (int m = 0; m <= JCI_SIGMA_GEMM32; m++) times[m] = t[m];
     return 0;
 }
 
@@ -836,6 +1010,12 @@ static int sigma_block(Work &w, int first, int cnt, int method)
     double t0 = now_s();
     for (int b = first; b < first + cnt; b += JCI_MAXK) {
         int k = std::min(JCI_MAXK, first + cnt - b);
+        if (w.op->sigma_cols) {
+            int rc = w.op->sigma_cols(w.op->self, method, k, w.V + (size_t)b * w.ld, w.ld,
+                                      w.S + (size_t)b * w.ld, w.ld);
+            if (rc < 0) return rc;
+            if (rc == 0) continue;
+        }
         RCHECK(jk_pack(k, w.N, w.V + (size_t)b * w.ld, w.ld, w.X, w.c->stream));
         RCHECK(w.op->sigma(w.op->self, method, k, w.X, w.Y));
         RCHECK(jk_unpack(k, w.N, w.Y, w.S + (size_t)b * w.ld, w.ld, w.c->stream));
@@ -966,9 +1146,14 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     JCI_CHECK(hipSetDevice(c->device));
     long long N = op->N;
     int nroots = p->nroots, krymax = std::min<long long>(p->krymax, N);
-    int krymin = std::max(p->krymin, nroots);
     if (nroots < 1 || nroots > JCI_MAXROOTS || nroots > N) { jci_set_error("solve: nroots out of range"); return -1; }
     if (krymax > JCI_MAXBASIS) { jci_set_error("solve: krymax above %d", JCI_MAXBASIS); return -1; }
+    if (p->nbuffer < 0) { jci_set_error("solve: nbuffer must be >= 0"); return -1; }
+    // tracked Ritz pairs: the requested roots plus the buffer ones (space permitting)
+    const long long nt_cap = std::max<long long>(nroots, krymax / 2);
+    const int ntrack = (int)std::min(std::min<long long>(nroots + p->nbuffer, JCI_MAXROOTS),
+                                     std::min<long long>(N, nt_cap));
+    int krymin = std::max(p->krymin, ntrack);
     if (krymin > krymax) krymin = krymax;
     if (krymin < nroots) { jci_set_error("solve: krymax below nroots"); return -1; }
     Work w{};
@@ -1021,38 +1206,56 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             for (int j = 0; j < i; j++) G[i * m + j] = G[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
         Yf.assign((size_t)m * m, 0.0);
         jacobi_eig(m, G.data(), theta.data(), Yf.data());
-        // residuals of the lowest nroots Ritz pairs -> T (preconditioned), norms
-        std::vector<double> Ysel((size_t)m * nroots);
+        // residuals of the lowest ntrack Ritz pairs -> T (preconditioned), norms
+        const int nt = std::min(ntrack, m);
+        std::vector<double> Ysel((size_t)m * nt);
         for (int i = 0; i < m; i++)
-            for (int r = 0; r < nroots; r++) Ysel[(size_t)i * nroots + r] = Yf[(size_t)r * m + i];
+            for (int r = 0; r < nt; r++) Ysel[(size_t)i * nt + r] = Yf[(size_t)r * m + i];
         double *d_th;
-        // Ysel and theta share the small scratch: Ysel at [0, m*nroots), theta after it
+        // Ysel and theta share the small scratch: Ysel at [0, m*nt), theta after it
         std::vector<double> packed(Ysel);
-        packed.insert(packed.end(), theta.begin(), theta.begin() + nroots);
+        packed.insert(packed.end(), theta.begin(), theta.begin() + nt);
         if ((rc = upload_small(w, packed.data(), (int)packed.size(), &dY))) goto done;
-        d_th = dY + (size_t)m * nroots;
-        if ((rc = jk_residual(m, nroots, N, w.V, w.S, w.ld, dY, d_th, op->d_hd, w.T,
+        d_th = dY + (size_t)m * nt;
+        if ((rc = jk_residual(m, nt, N, w.V, w.S, w.ld, dY, d_th, op->d_hd, w.T,
                               c->d_partial, JCI_PARTIAL_BLOCKS, c->d_small + 2048, c->stream)))
             goto done;
-        if (hipMemcpyAsync(c->h_small, c->d_small + 2048, sizeof(double) * nroots,
+        if (hipMemcpyAsync(c->h_small, c->d_small + 2048, sizeof(double) * nt,
                            hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess) { rc = -2; goto done; }
         double rmax = 0.0;
-        for (int r = 0; r < nroots; r++) { rn[r] = sqrt(c->h_small[r]); rmax = std::max(rmax, rn[r]); }
+        std::vector<double> rall(nt);
+        for (int r = 0; r < nt; r++) rall[r] = sqrt(c->h_small[r]);
+        for (int r = 0; r < nroots; r++) { rn[r] = rall[r]; rmax = std::max(rmax, rn[r]); }
         if (p->verbose)
             fprintf(stdout, " iter %3d  m %2d  %s  E0 %.12f  max|r| %.3e\n", it, m,
                     phase32 ? "fp32" : "fp64", theta[0] + p->totfrze, rmax);
         if (phase32 && (rmax < p->mixed_switch || it == p->maxiter)) {
-            // hand over to FP64: recompute sigma of the whole basis in FP64
+            // hand over to FP64: keep the lowest kc Ritz vectors of the FP32 phase (the
+            // basis itself is FP64) and recompute their sigma in FP64
             phase32 = 0;
+            int kc = std::min(m, krymin);
+            if (kc < m) {
+                std::vector<double> Yc((size_t)m * kc);
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < kc; j++) Yc[(size_t)i * kc + j] = Yf[(size_t)j * m + i];
+                if ((rc = upload_small(w, Yc.data(), m * kc, &dY))) goto done;
+                if ((rc = jk_rotate_inplace(m, kc, N, w.V, w.ld, dY, c->stream))) goto done;
+                m = kc;
+            }
             if ((rc = sigma_block(w, 0, m, cur_method()))) goto done;
             count_sigma(m);
             continue;
         }
-        std::vector<int> todo;
+        bool req_done = true;
         for (int r = 0; r < nroots; r++)
-            if (!(rn[r] < p->restol)) todo.push_back(r);
-        if (todo.empty()) { converged = 1; break; }
+            if (!(rn[r] < p->restol)) req_done = false;
+        if (req_done) { converged = 1; break; }
+        // corrections: every unconverged tracked pair (buffer pairs only while a requested
+        // root is still open, which is always the case here)
+        std::vector<int> todo;
+        for (int r = 0; r < nt; r++)
+            if (!(rall[r] < p->restol)) todo.push_back(r);
         if (it == p->maxiter) break;
         int k = (int)todo.size();
         // compact the unconverged corrections to the front of T
@@ -1062,9 +1265,10 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
                                hipMemcpyDeviceToDevice, c->stream) != hipSuccess) { rc = -2; goto done; }
         if (m + k > krymax) {
             // collapse onto the lowest kc Ritz vectors (basis and sigma rotated together)
-            int kc = std::max(krymin, nroots);
+            int kc = std::max(krymin, nt);
             kc = std::min(kc, krymax - k);
             if (kc < nroots) kc = nroots;
+            if (kc + k > krymax) k = krymax - kc;   // drop trailing buffer corrections
             std::vector<double> Yc((size_t)m * kc);
             for (int i = 0; i < m; i++)
                 for (int j = 0; j < kc; j++) Yc[(size_t)i * kc + j] = Yf[(size_t)j * m + i];
@@ -1120,6 +1324,17 @@ static int ham_sigma(void *self, int method, int k, const double *x, double *y)
     return sigma_dev((jci_ham *)self, method, k, x, y);
 }
 
+// vector-major sigma; returns 1 when the method has no such form (the caller packs)
+static int ham_sigma_cols(void *self, int method, int k, const double *V, long long ldv,
+                          double *S, long long lds)
+{
+    jci_ham *h = (jci_ham *)self;
+    method = resolve_method(h, method, 0);
+    if (!is_gf(method)) return 1;
+    if (!h->have_gf) { jci_set_error("sigma: dense-string sigma not applicable"); return -1; }
+    return gf_run(h, method == JCI_SIGMA_GEMM64 ? 64 : 32, k, V, ldv, S, lds);
+}
+
 static int ham_masks(void *self, long long n, const long long *d_idx, uint64_t *oa, uint64_t *ob)
 {
     jci_ham *h = (jci_ham *)self;
@@ -1135,7 +1350,7 @@ static int ham_resolve(void *self, int method, int want32)
 JCI_API int jci_solve(jci_ham *h, const jci_solve_params *p, const double *guess,
                          double *evals, double *evecs, jci_solve_stats *stats)
 {
-    SolveOp op;
+    SolveOp op{};
     op.self = h;
     op.ctx = h->sp->ctx;
     op.N = h->sp->h.ndets;
@@ -1143,6 +1358,7 @@ JCI_API int jci_solve(jci_ham *h, const jci_solve_params *p, const double *guess
     op.d_m1 = h->d_m1;
     op.d_m2 = h->d_m2;
     op.sigma = ham_sigma;
+    op.sigma_cols = ham_sigma_cols;
     op.masks = ham_masks;
     op.resolve = ham_resolve;
     return jci_solve_core(&op, p, guess, evals, evecs, stats);
@@ -1162,7 +1378,7 @@ jci_ctx *jci_ham_ctx(jci_ham *h) { return h->sp->ctx; }
 // initial guess block of a Hamiltonian into V (nguess vectors, leading dimension ld)
 int jci_ham_guess_block(jci_ham *h, int refdim, int nguess, int diag_guess, double *V, long long ld)
 {
-    SolveOp op;
+    SolveOp op{};
     op.self = h;
     op.ctx = h->sp->ctx;
     op.N = h->sp->h.ndets;
@@ -1170,6 +1386,7 @@ int jci_ham_guess_block(jci_ham *h, int refdim, int nguess, int diag_guess, doub
     op.d_m1 = h->d_m1;
     op.d_m2 = h->d_m2;
     op.sigma = ham_sigma;
+    op.sigma_cols = ham_sigma_cols;
     op.masks = ham_masks;
     op.resolve = ham_resolve;
     Work w{};
@@ -1236,5 +1453,13 @@ JCI_API int jci_synth_integrals(int n, uint64_t seed, double *m1, double *m2)
             for (int P = 0; P < np; P++) s += L[(size_t)P * n1 + a] * L[(size_t)P * n1 + b];
             m2[a * (a + 1) / 2 + b] = s;
         }
+    return 0;
+}
+
+// dense-string sigma constants of a Hamiltonian (1 when it has none)
+int jci_ham_gemmfci(jci_ham *h, GemmFci *out)
+{
+    if (!h->have_gf) return 1;
+    *out = h->gf;
     return 0;
 }

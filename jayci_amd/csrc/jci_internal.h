@@ -69,6 +69,10 @@ struct SolveOp {
     const double *d_hd, *d_m1, *d_m2;
     // y = H x, determinant-major [N][k] device arrays
     int (*sigma)(void *self, int method, int k, const double *x, double *y);
+    // optional: y = H x on vector-major columns (V + j*ldv -> S + j*lds); returns 1 when the
+    // method has no such form and the caller must pack to determinant-major
+    int (*sigma_cols)(void *self, int method, int k, const double *V, long long ldv, double *S,
+                      long long lds);
     // masks of determinants d_idx[0..n) into device arrays oa, ob
     int (*masks)(void *self, long long n, const long long *d_idx, uint64_t *oa, uint64_t *ob);
     // concrete sigma method for a requested one (0 = automatic) and precision

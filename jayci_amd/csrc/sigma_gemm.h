@@ -1,0 +1,80 @@
+// Dense-string sigma for full-CI spaces (one alpha group, one beta group, one block).
+//
+// With X_k the k-th vector viewed as an Na x Nb matrix (alpha string rows, beta string
+// columns, determinant index Ia*Nb + Ib), the Hamiltonian splits into
+//
+//   sigma_k = Haa X_k + X_k Hbb^T + sum_{P,Q} (P|Q) A_P X_k B_Q^T
+//
+// Haa / Hbb: same-spin string Hamiltonians (one-electron part, same-spin two-electron part
+// and the same-spin diagonal), held dense: for the spaces this path takes (n <= 15 orbitals)
+// they are 10-30 % filled, so an MFMA product beats a sparse walk.  A_P / B_Q: the
+// single-replacement operators E_ij + E_ji of an orbital pair P = tri(i,j) on alpha / beta
+// strings, including the diagonal pairs (E_kk = occupation of k), which carry the
+// opposite-spin Coulomb terms of the diagonal and of the singles.  The opposite-spin part
+// is evaluated as
+//
+//   D[Q][k,Ja,Ib] = sum_{Jb : <Ib|B_Q|Jb> != 0} sign * X_k[Ja][Jb]        (beta lists)
+//   E[P][k,Ja,Ib] = sum_Q (P|Q) D[Q][k,Ja,Ib]                            (MFMA, D in LDS)
+//   sigma_k[Ia][Ib] += sum_{(Ja,P,sign) in alpha list of Ia} sign * E[P][k,Ja,Ib]
+//
+// Matrix elements are those of the Slater-Condon rules in slater.h (the corrected
+// same-spin doubles of mpi_source/action_util.c:215-240).
+#pragma once
+#include "jci_internal.h"
+
+#define GF_MAXMP 128          // padded orbital-pair count (n <= 15)
+#define GF_MAXSTR 65535       // strings per spin addressable by a 16-bit list entry
+
+// list entry: bits 0..15 target string, 16..23 orbital pair, bit 31 sign
+struct GemmFci {
+    long long na, nb;            // alpha / beta strings
+    int m1, MP;                  // orbital pairs, padded to a multiple of 16
+    int La, Lb;                  // list entries per alpha / beta string
+    const double *H64[2];        // dense same-spin Hamiltonians, [Na][Na] and [Nb][Nb]
+    const float *H32[2];
+    const double *M64;           // (P|Q), MP x MP, zero padded
+    const float *M32;
+    const uint32_t *la, *lb;     // [Na][La], [Nb][Lb]
+};
+
+// One instance's sigma in a grouped launch.  Every kernel of the sigma walks a flattened
+// space of work units (FP32 conversion blocks, same-spin product tiles, E column tiles,
+// gather blocks); base[] holds the first unit of the task in each space, and the header
+// the totals, so one persistent launch per kernel covers any number of instances.
+struct GfTask {
+    GemmFci g;
+    int K;                       // vectors
+    const double *x64;           // FP64 input columns, vector k at x64 + k*ldx64
+    long long ldx64;
+    float *x32;                  // FP32 copies [K][N] (FP32 launches)
+    double *y;                   // output columns, vector k at y + k*ldy (overwritten)
+    long long ldy;
+    void *E;                     // intermediate, m1 rows of K*N elements
+    long long base[4];           // first unit: conversion, same-spin, E, gather
+};
+struct GfHeader {
+    int n;                       // tasks
+    long long total[4];          // units per kernel
+};
+#define GF_CV 0
+#define GF_SS 1
+#define GF_E 2
+#define GF_GA 3
+
+// work units of one task per kernel for the given precision
+void jk_gf_units(int prec, const GfTask &t, long long units[4]);
+// the grouped sigma: (FP32: conversion) -> E -> same-spin product -> gather.  hdr_host
+// (may be NULL) sizes the grids to the work; NULL launches the fixed persistent grids
+// that read the totals from d_hdr (graph capture).
+int jk_gf_group(int prec, const GfHeader *d_hdr, const GfTask *d_tasks,
+                const GfHeader *hdr_host, hipStream_t st);
+
+// build the dense same-spin matrix of one spin from the excitation tables
+int jk_gf_dense(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
+                const double *v1, const long long *off2, const int2 *e2, const double *v2,
+                const double *m1, const double *m2, double *H, hipStream_t st);
+// replacement lists (singles + diagonal pairs) of one spin
+int jk_gf_lists(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
+                int L, uint32_t *out, hipStream_t st);
+// (P|Q) padded to MP x MP in FP64 and FP32
+int jk_gf_m2pad(int m1, int MP, const double *m2sq, double *o64, float *o32, hipStream_t st);

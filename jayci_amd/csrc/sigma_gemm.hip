@@ -1,0 +1,432 @@
+// Dense-string full-CI sigma on MFMA (see sigma_gemm.h for the decomposition).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "sigma_gemm.h"
+#include "slater.h"
+
+#define RCHECK(x)                      \
+    do {                               \
+        int rc_ = (x);                 \
+        if (rc_) return rc_;           \
+    } while (0)
+
+#define GF_CHECK()                                                       \
+    do {                                                                 \
+        hipError_t e_ = hipGetLastError();                               \
+        if (e_ != hipSuccess) {                                          \
+            jci_set_error("%s:%d launch: %s", __FILE__, __LINE__,        \
+                          hipGetErrorString(e_));                        \
+            return -2;                                                   \
+        }                                                                \
+    } while (0)
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// 16x16 output tile, 4-deep reduction per instruction.  Operand lanes: A[l % 16][l / 16],
+// B[l / 16][l % 16]; accumulator element v of lane l sits in column l % 16 and row row(l, v).
+template <typename T> struct Mf;
+template <> struct Mf<double> {
+    typedef v4d acc;
+    static __device__ __forceinline__ acc zero() { return (v4d){0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc mma(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int v) { return l / 16 + 4 * v; }
+};
+template <> struct Mf<float> {
+    typedef v4f acc;
+    static __device__ __forceinline__ acc zero() { return (v4f){0.0f, 0.0f, 0.0f, 0.0f}; }
+    static __device__ __forceinline__ acc mma(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int v) { return 4 * (l / 16) + v; }
+};
+
+// ---- set-up kernels ---------------------------------------------------------------------------
+__global__ void k_gf_dense(long long nstr, const uint64_t *__restrict__ mask,
+                           const long long *__restrict__ off1, const int2 *__restrict__ e1,
+                           const double *__restrict__ v1, const long long *__restrict__ off2,
+                           const int2 *__restrict__ e2, const double *__restrict__ v2,
+                           const double *__restrict__ m1, const double *__restrict__ m2,
+                           double *__restrict__ H)
+{
+    long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nstr) return;
+    const uint64_t s = mask[I];
+    double d = 0.0;
+    for (uint64_t x = s; x; x &= x - 1) {
+        int i = jci_ctz(x);
+        d += jci_h1(m1, i, i);
+        for (uint64_t y = s & ((1ULL << i) - 1ULL); y; y &= y - 1) {
+            int j = jci_ctz(y);
+            d += jci_eri(m2, i, i, j, j) - jci_eri(m2, i, j, i, j);
+        }
+    }
+    double *row = H + I * nstr;
+    row[I] = d;
+    for (long long e = off1[I]; e < off1[I + 1]; e++) {
+        int2 en = e1[e];
+        double v = v1[e];
+        row[en.x] = (en.y & (1 << 12)) ? -v : v;
+    }
+    for (long long e = off2[I]; e < off2[I + 1]; e++) row[e2[e].x] = v2[e];
+}
+
+__global__ void k_gf_lists(long long nstr, const uint64_t *__restrict__ mask,
+                           const long long *__restrict__ off1, const int2 *__restrict__ e1, int L,
+                           uint32_t *__restrict__ out)
+{
+    long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nstr) return;
+    uint32_t *o = out + I * L;
+    int n = 0;
+    for (long long e = off1[I]; e < off1[I + 1] && n < L; e++) {
+        int2 en = e1[e];
+        int i = en.y & 63, f = (en.y >> 6) & 63;
+        uint32_t P = (uint32_t)jci_tri(i, f);
+        o[n++] = (uint32_t)en.x | (P << 16) | ((uint32_t)((en.y >> 12) & 1) << 31);
+    }
+    for (uint64_t x = mask[I]; x && n < L; x &= x - 1) {
+        int k = jci_ctz(x);
+        o[n++] = (uint32_t)I | ((uint32_t)jci_tri(k, k) << 16);
+    }
+}
+
+__global__ void k_gf_m2pad(int m1, int MP, const double *__restrict__ sq, double *__restrict__ o64,
+                           float *__restrict__ o32)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= MP * MP) return;
+    int P = idx / MP, Q = idx - P * MP;
+    double v = (P < m1 && Q < m1) ? sq[(long long)P * m1 + Q] : 0.0;
+    o64[idx] = v;
+    o32[idx] = (float)v;
+}
+
+
+// ---- grouped kernels ------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ const T *hmat(const GemmFci &g, int spin);
+template <> __device__ __forceinline__ const double *hmat<double>(const GemmFci &g, int spin)
+{
+    return g.H64[spin];
+}
+template <> __device__ __forceinline__ const float *hmat<float>(const GemmFci &g, int spin)
+{
+    return g.H32[spin];
+}
+template <typename T> __device__ __forceinline__ const T *mpad(const GemmFci &g);
+template <> __device__ __forceinline__ const double *mpad<double>(const GemmFci &g) { return g.M64; }
+template <> __device__ __forceinline__ const float *mpad<float>(const GemmFci &g) { return g.M32; }
+
+// input vector k of a task in the arithmetic type
+template <typename T> __device__ __forceinline__ const T *xin(const GfTask &t, int k);
+template <> __device__ __forceinline__ const double *xin<double>(const GfTask &t, int k)
+{
+    return t.x64 + (long long)k * t.ldx64;
+}
+template <> __device__ __forceinline__ const float *xin<float>(const GfTask &t, int k)
+{
+    return t.x32 + (long long)k * t.g.na * t.g.nb;
+}
+
+// task owning flattened unit u of kernel `which` (tasks ordered by base)
+__device__ __forceinline__ int find_task(const GfTask *__restrict__ T, int n, long long u, int which)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (T[mid].base[which] <= u) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// FP64 -> FP32 copies of the input vectors, 256 elements per unit
+__global__ __launch_bounds__(256) void k_gcv(const GfHeader *__restrict__ H,
+                                             const GfTask *__restrict__ tasks)
+{
+    const int n = H->n;
+    const long long total = H->total[GF_CV];
+    for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+        const GfTask &t = tasks[find_task(tasks, n, u, GF_CV)];
+        const long long N = t.g.na * t.g.nb;
+        const long long idx = (u - t.base[GF_CV]) * 256 + threadIdx.x;
+        if (idx < N * t.K) {
+            const long long k = idx / N, i = idx - k * N;
+            t.x32[idx] = (float)t.x64[k * t.ldx64 + i];
+        }
+    }
+}
+
+// C(64x64 tile at m0, n0) += A B over Kr, A row-major (lda), B row-major (ldb), M x N bounds
+template <typename T>
+__device__ __forceinline__ void mma_tile(typename Mf<T>::acc (&acc)[2][2], const T *__restrict__ A,
+                                         long long lda, const T *__restrict__ B, long long ldb,
+                                         long long M, long long N, long long Kr, long long m0,
+                                         long long n0, T (*As)[65], T (*Bs)[65])
+{
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int ar = tid >> 2, ak = (tid & 3) * 4;
+    const int bk = tid >> 4, bc = (tid & 15) * 4;
+    for (long long k0 = 0; k0 < Kr; k0 += 16) {
+        const long long gm = m0 + ar;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long long gk = k0 + ak + j;
+            As[ak + j][ar] = (gm < M && gk < Kr) ? A[gm * lda + gk] : (T)0;
+        }
+        const long long gk = k0 + bk;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long long gn = n0 + bc + j;
+            Bs[bk][bc + j] = (gk < Kr && gn < N) ? B[gk * ldb + gn] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const int kr = ks * 4 + l / 16;
+            const T a0 = As[kr][wm * 32 + l % 16], a1 = As[kr][wm * 32 + 16 + l % 16];
+            const T b0 = Bs[kr][wn * 32 + l % 16], b1 = Bs[kr][wn * 32 + 16 + l % 16];
+            acc[0][0] = Mf<T>::mma(a0, b0, acc[0][0]);
+            acc[0][1] = Mf<T>::mma(a0, b1, acc[0][1]);
+            acc[1][0] = Mf<T>::mma(a1, b0, acc[1][0]);
+            acc[1][1] = Mf<T>::mma(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+}
+
+// same-spin parts: y_k = Haa X_k + X_k Hbb on 64 x 64 tiles (k, alpha rows, beta columns)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gss(const GfHeader *__restrict__ H,
+                                             const GfTask *__restrict__ tasks)
+{
+    __shared__ T As[16][65];
+    __shared__ T Bs[16][65];
+    const int n = H->n;
+    const long long total = H->total[GF_SS];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+        const GfTask &t = tasks[find_task(tasks, n, u, GF_SS)];
+        const GemmFci &g = t.g;
+        const long long na = g.na, nb = g.nb;
+        const long long tn = (nb + 63) / 64, tm = (na + 63) / 64;
+        const long long loc = u - t.base[GF_SS];
+        const int k = (int)(loc / (tm * tn));
+        const long long rem = loc - (long long)k * tm * tn;
+        const long long m0 = (rem / tn) * 64, n0 = (rem % tn) * 64;
+        const T *X = xin<T>(t, k);
+        typename Mf<T>::acc acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = Mf<T>::zero();
+        mma_tile<T>(acc, hmat<T>(g, 0), na, X, nb, na, nb, na, m0, n0, As, Bs);
+        mma_tile<T>(acc, X, nb, hmat<T>(g, 1), nb, na, nb, nb, m0, n0, As, Bs);
+        double *y = t.y + (long long)k * t.ldy;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const long long r = m0 + wm * 32 + i * 16 + Mf<T>::row(l, v);
+                    const long long c = n0 + wn * 32 + j * 16 + l % 16;
+                    if (r < na && c < nb) y[r * nb + c] = (double)acc[i][j][v];
+                }
+    }
+}
+
+// E[P][c] = sum_Q (P|Q) D[Q][c] over columns c = (k, Ja, Ib): the tile of D is built in LDS
+// from the beta lists (each (Q, column) receives at most one entry) and multiplied by the
+// (P|Q) fragments that every wave keeps in registers (wave w: row tiles w and w + 4),
+// reloaded when the task changes.
+template <typename T, int BN>
+__global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
+                                            const GfTask *__restrict__ tasks)
+{
+    __shared__ T Ds[GF_MAXMP][BN + 1];
+    constexpr int NCT = BN / 16, PARTS = 256 / BN, KSMAX = GF_MAXMP / 4;
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    const int n = H->n;
+    const long long total = H->total[GF_E];
+    const int col = tid % BN, part = tid / BN;
+    T a[2][KSMAX];
+    int cur = -1;
+    for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+        const int ti = find_task(tasks, n, u, GF_E);
+        const GfTask &t = tasks[ti];
+        const GemmFci &g = t.g;
+        const int MP = g.MP, KS = MP / 4, RT = MP / 16;
+        if (ti != cur) {
+            const T *M2 = mpad<T>(g);
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int rt = w + 4 * i;
+#pragma unroll
+                for (int ks = 0; ks < KSMAX; ks++)
+                    a[i][ks] = (rt < RT && ks < KS)
+                                   ? M2[(long long)(rt * 16 + l % 16) * MP + ks * 4 + l / 16]
+                                   : (T)0;
+            }
+            cur = ti;
+        }
+        const long long Nb = g.nb, N = g.na * g.nb, ncols = N * t.K;
+        const long long c0 = (u - t.base[GF_E]) * BN;
+        for (int idx = tid; idx < MP * BN; idx += 256) Ds[idx / BN][idx % BN] = (T)0;
+        __syncthreads();
+        const long long c = c0 + col;
+        if (c < ncols) {
+            const long long k = c / N, rem = c - k * N;
+            const long long Ja = rem / Nb, Ib = rem - Ja * Nb;
+            const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
+            const uint32_t *__restrict__ lst = g.lb + Ib * g.Lb;
+            for (int e = part; e < g.Lb; e += PARTS) {
+                const uint32_t wd = lst[e];
+                const T v = xr[wd & 0xFFFFu];
+                Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
+            }
+        }
+        __syncthreads();
+        typename Mf<T>::acc acc[2][NCT];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < NCT; j++) acc[i][j] = Mf<T>::zero();
+#pragma unroll
+        for (int ks = 0; ks < KSMAX; ks++) {
+            if (ks < KS) {
+                T b[NCT];
+#pragma unroll
+                for (int j = 0; j < NCT; j++) b[j] = Ds[ks * 4 + l / 16][j * 16 + l % 16];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    if (w + 4 * i < RT) {
+#pragma unroll
+                        for (int j = 0; j < NCT; j++) acc[i][j] = Mf<T>::mma(a[i][ks], b[j], acc[i][j]);
+                    }
+            }
+        }
+        T *__restrict__ E = (T *)t.E;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int rt = w + 4 * i;
+            if (rt >= RT) continue;
+#pragma unroll
+            for (int j = 0; j < NCT; j++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int P = rt * 16 + Mf<T>::row(l, v);
+                    const long long cc = c0 + j * 16 + l % 16;
+                    if (P < g.m1 && cc < ncols) E[(long long)P * ncols + cc] = acc[i][j][v];
+                }
+        }
+        __syncthreads();
+    }
+}
+
+// y_k[Ia][Ib] += sum over the alpha list of Ia of sign * E[P][k, Ja, Ib]; 256 elements per unit
+template <typename T>
+__global__ __launch_bounds__(256) void k_gga(const GfHeader *__restrict__ H,
+                                             const GfTask *__restrict__ tasks)
+{
+    const int n = H->n;
+    const long long total = H->total[GF_GA];
+    for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+        const GfTask &t = tasks[find_task(tasks, n, u, GF_GA)];
+        const GemmFci &g = t.g;
+        const long long Nb = g.nb, N = g.na * g.nb, ncols = N * t.K;
+        const long long idx = (u - t.base[GF_GA]) * 256 + threadIdx.x;
+        if (idx >= ncols) continue;
+        const long long k = idx / N, rem = idx - k * N;
+        const long long Ia = rem / Nb, Ib = rem - Ia * Nb;
+        const uint32_t *__restrict__ lst = g.la + Ia * g.La;
+        const T *__restrict__ Ek = (const T *)t.E + k * N + Ib;
+        T acc = (T)0;
+        for (int e = 0; e < g.La; e++) {
+            const uint32_t wd = lst[e];
+            const T v = Ek[(long long)((wd >> 16) & 0xFFu) * ncols + (long long)(wd & 0xFFFFu) * Nb];
+            acc += (wd >> 31) ? -v : v;
+        }
+        t.y[k * t.ldy + rem] += (double)acc;
+    }
+}
+
+// ---- launchers ------------------------------------------------------------------------------
+int jk_gf_dense(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
+                const double *v1, const long long *off2, const int2 *e2, const double *v2,
+                const double *m1, const double *m2, double *H, hipStream_t st)
+{
+    if (hipMemsetAsync(H, 0, sizeof(double) * (size_t)nstr * nstr, st) != hipSuccess) {
+        jci_set_error("gf_dense: memset failed");
+        return -2;
+    }
+    hipLaunchKernelGGL(k_gf_dense, dim3((unsigned)((nstr + 127) / 128)), dim3(128), 0, st, nstr,
+                       mask, off1, e1, v1, off2, e2, v2, m1, m2, H);
+    GF_CHECK();
+    return 0;
+}
+
+int jk_gf_lists(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
+                int L, uint32_t *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gf_lists, dim3((unsigned)((nstr + 127) / 128)), dim3(128), 0, st, nstr,
+                       mask, off1, e1, L, out);
+    GF_CHECK();
+    return 0;
+}
+
+int jk_gf_m2pad(int m1, int MP, const double *m2sq, double *o64, float *o32, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gf_m2pad, dim3((MP * MP + 255) / 256), dim3(256), 0, st, m1, MP, m2sq,
+                       o64, o32);
+    GF_CHECK();
+    return 0;
+}
+
+#define GF_BN64 32
+#define GF_BN32 64
+
+void jk_gf_units(int prec, const GfTask &t, long long units[4])
+{
+    const long long na = t.g.na, nb = t.g.nb, N = na * nb, cols = N * t.K;
+    const int bn = prec == 64 ? GF_BN64 : GF_BN32;
+    units[GF_CV] = prec == 64 ? 0 : (cols + 255) / 256;
+    units[GF_SS] = (long long)t.K * ((na + 63) / 64) * ((nb + 63) / 64);
+    units[GF_E] = (cols + bn - 1) / bn;
+    units[GF_GA] = (cols + 255) / 256;
+}
+
+// persistent grid caps per kernel
+static const long long gf_cap[4] = {4096, 2048, 2048, 4096};
+
+template <typename T, int BN>
+static int gf_group_t(const GfHeader *d_hdr, const GfTask *d_tasks, const GfHeader *hh,
+                      hipStream_t st)
+{
+    unsigned g[4];
+    for (int i = 0; i < 4; i++)
+        g[i] = (unsigned)(hh ? std::max<long long>(1, std::min(hh->total[i], gf_cap[i])) : gf_cap[i]);
+    if (sizeof(T) == 4) {
+        hipLaunchKernelGGL(k_gcv, dim3(g[GF_CV]), dim3(256), 0, st, d_hdr, d_tasks);
+        GF_CHECK();
+    }
+    hipLaunchKernelGGL((k_gE<T, BN>), dim3(g[GF_E]), dim3(256), 0, st, d_hdr, d_tasks);
+    GF_CHECK();
+    hipLaunchKernelGGL(k_gss<T>, dim3(g[GF_SS]), dim3(256), 0, st, d_hdr, d_tasks);
+    GF_CHECK();
+    hipLaunchKernelGGL(k_gga<T>, dim3(g[GF_GA]), dim3(256), 0, st, d_hdr, d_tasks);
+    GF_CHECK();
+    return 0;
+}
+
+int jk_gf_group(int prec, const GfHeader *d_hdr, const GfTask *d_tasks, const GfHeader *hdr_host,
+                hipStream_t st)
+{
+    if (prec == 64) return gf_group_t<double, GF_BN64>(d_hdr, d_tasks, hdr_host, st);
+    return gf_group_t<float, GF_BN32>(d_hdr, d_tasks, hdr_host, st);
+}

@@ -45,6 +45,15 @@ def ndets_of(spec):
     return None
 
 
+def solve_params(spec, N, restol=1e-6, nbuffer=1):
+    """Davidson settings of a request (the same for the served and the one-at-a-time path):
+    block of nroots + nbuffer tracked pairs, subspace up to 4x that (at least 12)."""
+    nr = spec["nroots"]
+    nt = nr + nbuffer
+    return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N), krymax=min(max(4 * nt, 12), N),
+                refdim=min(64, N), guess=GUESS_DIAG, totfrze=SYNTH_ENUC, nbuffer=nbuffer)
+
+
 class Request:
     __slots__ = ("idx", "arrival", "spec", "seed", "space", "ham", "slot", "t_admit", "t_done",
                  "energies", "stats", "bytes")
@@ -62,14 +71,16 @@ def percentile(xs, q):
 
 class Server:
     def __init__(self, ctx, capacity=64, budget_bytes=200e9, max_norbs=16, precision=PREC_FP64,
-                 restol=1e-6, mixed_switch=1e-4, graphs=True):
+                 restol=1e-6, mixed_switch=1e-4, graphs=True, nbuffer=1, pool_bytes=24 << 30):
         self.ctx = ctx
         self.capacity = capacity
-        self.budget = budget_bytes
+        # the sigma pool of the opposite-spin intermediates is part of the budget
+        self.budget = budget_bytes - pool_bytes
         self.precision = precision
         self.restol = restol
         self.mixed_switch = mixed_switch
-        self.batch = Batch(ctx, capacity, max_norbs, graphs=graphs)
+        self.nbuffer = nbuffer
+        self.batch = Batch(ctx, capacity, max_norbs, graphs=graphs, pool_bytes=pool_bytes)
 
     def close(self):
         self.batch.close()
@@ -78,15 +89,12 @@ class Server:
         sp = req.spec
         nr = sp["nroots"]
         N = req.space.ndets
-        return req.ham.params(nroots=nr, restol=self.restol, krymin=min(nr + 1, N),
-                              krymax=min(max(nr + 1 + 2 * nr, 12), N), refdim=min(64, N),
-                              precision=self.precision, mixed_switch=self.mixed_switch,
-                              guess=GUESS_DIAG, totfrze=SYNTH_ENUC)
+        return req.ham.params(**solve_params(sp, N, self.restol, self.nbuffer),
+                              precision=self.precision, mixed_switch=self.mixed_switch)
 
     def _estimate_bytes(self, spec):
         N = ndets_of(spec) or 10 ** 6
-        nr = spec["nroots"]
-        krymax = max(nr + 1 + 2 * nr, 12)
+        krymax = solve_params(spec, N, self.restol, self.nbuffer)["krymax"]
         return Batch.bytes_for(N, krymax) + 64 * N
 
     def run(self, stream, time_scale=1.0):

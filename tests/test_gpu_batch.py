@@ -79,3 +79,30 @@ def test_batch_continuous_admission(J, orc, ctx):
         alone, _, _ = h.solve(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
                               refdim=min(48, sp.ndets), precision=J.PREC_MIXED, guess=J.GUESS_DIAG)
         np.testing.assert_allclose(results[i], alone, rtol=0, atol=1e-8)
+
+
+def test_batch_pool_rounds_and_buffer_roots(J, orc, ctx):
+    """A pool smaller than two vectors' intermediates forces one vector per sigma round; with
+    buffer roots the results still match the standalone solves and the dense spectrum."""
+    specs = [(3, 3, 8, 0, 8, 0, 1, 2), (4, 4, 9, 0, 9, 0, 1, 1), (3, 2, 7, 0, 7, 0, 1, 2),
+             (3, 3, 8, 2, 2, 2, 1, 1)]
+    items = [_build(J, orc, ctx, spec, 300 + i) for i, spec in enumerate(specs)]
+    b = J.Batch(ctx, 8, 16, pool_bytes=2 << 20)
+    slots = []
+    for sp, h, w, nr in items:
+        p = h.params(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
+                     refdim=min(48, sp.ndets), precision=J.PREC_MIXED, guess=J.GUESS_DIAG,
+                     nbuffer=1)
+        slots.append(b.add(h, p))
+    b.run(400)
+    for (sp, h, w, nr), s in zip(items, slots):
+        ev, _, st = b.result(s)
+        assert st["done"] == 1, st
+        alone, _, _ = h.solve(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
+                              refdim=min(48, sp.ndets), precision=J.PREC_MIXED,
+                              guess=J.GUESS_DIAG, nbuffer=1)
+        np.testing.assert_allclose(ev, alone, rtol=0, atol=1e-8)
+        if w is not None:
+            np.testing.assert_allclose(ev, w[:nr], rtol=0, atol=1e-8)
+        b.release(s)
+    b.close()

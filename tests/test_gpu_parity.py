@@ -59,8 +59,10 @@ def test_rows_bitexact(J, orc, ctx, spec):
         np.testing.assert_allclose(v, ov, rtol=1e-12, atol=1e-13)
 
 
-METHODS64 = ["SIGMA_PAIRS64", "SIGMA_PAIRS_SYM64", "SIGMA_DIRECT64", "SIGMA_FAST64", "SIGMA_FCI64"]
-METHODS32 = ["SIGMA_PAIRS32", "SIGMA_DIRECT32", "SIGMA_FAST32", "SIGMA_FCI32"]
+METHODS64 = ["SIGMA_PAIRS64", "SIGMA_PAIRS_SYM64", "SIGMA_DIRECT64", "SIGMA_FAST64", "SIGMA_FCI64",
+             "SIGMA_GEMM64"]
+METHODS32 = ["SIGMA_PAIRS32", "SIGMA_DIRECT32", "SIGMA_FAST32", "SIGMA_FCI32", "SIGMA_GEMM32"]
+FCI_ONLY = ("SIGMA_FCI64", "SIGMA_FCI32", "SIGMA_GEMM64", "SIGMA_GEMM32")
 
 
 @pytest.mark.parametrize("spec", SMALL_SPACES)
@@ -74,7 +76,7 @@ def test_sigma_all_methods(J, orc, ctx, spec):
     scale = np.abs(ref).max()
     fci = spec[3] == 0 and spec[4] == spec[2]
     for name in METHODS64:
-        if name == "SIGMA_FCI64" and not fci:
+        if name in FCI_ONLY and not fci:
             continue
         meth = getattr(J, name)
         h = upper if name == "SIGMA_PAIRS_SYM64" else full
@@ -82,10 +84,27 @@ def test_sigma_all_methods(J, orc, ctx, spec):
         y = h.sigma(X[:k], method=meth)
         np.testing.assert_allclose(y, ref[:k], rtol=0, atol=1e-12 * scale, err_msg=name)
     for name in METHODS32:
-        if name == "SIGMA_FCI32" and not fci:
+        if name in FCI_ONLY and not fci:
             continue
         y = full.sigma(X, method=getattr(J, name))
         np.testing.assert_allclose(y, ref, rtol=0, atol=2e-5 * scale, err_msg=name)
+
+
+@pytest.mark.parametrize("spec,nvec", [((4, 4, 10, 0, 10, 0, 1), 1),   # 44,100 dets
+                                       ((5, 4, 11, 0, 11, 0, 1), 3),   # open shell, 152,460
+                                       ((5, 5, 11, 0, 11, 0, 1), 2)])  # 213,444
+def test_sigma_dense_string_larger(J, orc, ctx, spec, nvec):
+    """The dense-string MFMA sigma on full-CI spaces whose string counts and orbital-pair
+    counts span several 64-row product tiles and many column tiles of the intermediate."""
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec, seed=21)
+    h = J.Hamiltonian(gs, m1, m2, J.DIRECT_TABLES)
+    X = rand_vectors(gs.ndets, nvec, 8)
+    ref = os_.sigma(m1, m2, X)
+    scale = np.abs(ref).max()
+    y = h.sigma(X, method=J.SIGMA_GEMM64)
+    np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * scale)
+    y = h.sigma(X, method=J.SIGMA_GEMM32)
+    np.testing.assert_allclose(y, ref, rtol=0, atol=2e-5 * scale)
 
 
 def test_pair_counts_fci(J, orc, ctx):
@@ -122,12 +141,15 @@ def test_davidson_vs_dense(J, orc, ctx, spec):
     ("pairs", "SIGMA_PAIRS64", "SIGMA_PAIRS32"),
     ("none", "SIGMA_DIRECT64", "SIGMA_DIRECT32"),
     ("tables", "SIGMA_FAST64", "SIGMA_FAST32"),
+    ("gemm", "SIGMA_GEMM64", "SIGMA_GEMM32"),
 ])
 def test_davidson_precision_and_methods(J, orc, ctx, precision, flags, sig64, sig32):
-    gs, os_, m1, m2 = _spaces(J, orc, ctx, (3, 3, 8, 2, 2, 2, 1), seed=3)
+    spec = (3, 3, 8, 0, 8, 0, 1) if flags == "gemm" else (3, 3, 8, 2, 2, 2, 1)
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec, seed=3)
     H = os_.dense_h(m1, m2)
     w = np.linalg.eigh(H)[0]
-    fl = {"pairs": J.PAIRS_FP64 | J.PAIRS_FP32, "none": 0, "tables": J.DIRECT_TABLES}[flags]
+    fl = {"pairs": J.PAIRS_FP64 | J.PAIRS_FP32, "none": 0, "tables": J.DIRECT_TABLES,
+          "gemm": J.DIRECT_TABLES}[flags]
     h = J.Hamiltonian(gs, m1, m2, fl)
     ev, vec, st = h.solve(nroots=3, restol=1e-7, krymin=4, krymax=12, refdim=40,
                           precision=getattr(J, precision), sigma_fp64=getattr(J, sig64),
@@ -165,3 +187,18 @@ def test_gram_mfma_layout(J, orc, ctx):
         assert abs(vec[r] @ hv[r] - ev[r]) < 1e-10
         for q in range(r):
             assert abs(vec[r] @ vec[q]) < 1e-10
+
+
+def test_buffer_root_finds_lowest_state(J, ctx):
+    """Full CI 5a5b/10 (63,504 determinants, seed 400060) has its two lowest states 0.012 Ha
+    apart and a guess that favours the second; with one buffer root the mixed-precision solve
+    lands on the lowest.  Reference: Lanczos (scipy eigsh on the oracle sigma) gives
+    E0 = 36.10636502, E1 = 36.11837624 (E_nuc = 10 included)."""
+    m1, m2 = J.synth_integrals(10, 400060)
+    sp = J.Space(ctx, 5, 5, 10)
+    h = J.Hamiltonian(sp, m1, m2, J.DIRECT_TABLES)
+    for prec in (J.PREC_MIXED, J.PREC_FP64):
+        ev, _, st = h.solve(nroots=1, restol=1e-6, krymin=3, krymax=12, refdim=64,
+                            guess=J.GUESS_DIAG, totfrze=J.SYNTH_ENUC, vectors=False,
+                            precision=prec, nbuffer=1)
+        assert abs(ev[0] - 36.10636502) < 1e-7, (prec, ev, st)
