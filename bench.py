@@ -192,15 +192,21 @@ def run_plain(J, ctx, stream, restol, precision):
 
 
 def reference_energies(J, ctx, stream):
-    """Tight FP64 GPU solves (residual 1e-9, FP64 sigma) of every instance."""
-    ref = []
+    """Tight FP64 GPU solves (residual 1e-8, FP64 sigma, two buffer roots, up to 400
+    iterations) of every instance; the energy error of such a solve is ~|r|^2 / gap, far
+    below 1e-8 Ha.  Returns (energies, largest final residual norm)."""
+    ref, rmax = [], 0.0
     for arr, s, seed in stream:
         sp, h = make_instance(J, ctx, s, seed, J.DIRECT_TABLES)
-        ev, _, _ = h.solve(precision=J.PREC_FP64, **solve_kw(J, s, sp.ndets, 1e-9))
+        kw = solve_kw(J, s, sp.ndets, 1e-8)
+        kw.update(nbuffer=2, krymin=min(s["nroots"] + 3, sp.ndets),
+                  krymax=min(max(4 * (s["nroots"] + 2), 16), sp.ndets), maxiter=400)
+        ev, _, st = h.solve(precision=J.PREC_FP64, check=False, **kw)
         ref.append(list(ev))
+        rmax = max(rmax, max(st["rnorm"]))
         h.close()
         sp.close()
-    return ref
+    return ref, rmax
 
 
 def bench_stream(J, ctx, args, rate, seed):
@@ -210,6 +216,8 @@ def bench_stream(J, ctx, args, rate, seed):
     ndets = [serve.ndets_of(s) for _, s, _ in stream]
     log("stream: %d requests, %d determinants in total, largest %d"
         % (len(stream), sum(ndets), max(ndets)))
+
+    phases = {}
 
     def served(precision, st):
         srv = serve.Server(ctx, capacity=256, budget_bytes=200e9, max_norbs=16,
@@ -221,6 +229,7 @@ def bench_stream(J, ctx, args, rate, seed):
             ctx.sync()
             wall = time.perf_counter() - t0
             sig = srv.batch.sigma_seconds
+            phases[precision] = {k: round(v, 4) for k, v in srv.batch.phase_seconds.items()}
         finally:
             srv.close()
         return done, wall, sig
@@ -234,14 +243,23 @@ def bench_stream(J, ctx, args, rate, seed):
     log("plain fp64 one at a time: %.3f s" % wall_p)
     done_b, wall_b, sig_b = served(J.PREC_FP64, stream)
     log("served fp64: %.3f s" % wall_b)
-    ref = reference_energies(J, ctx, stream)
-    log("reference energies done")
+    ref, ref_r = reference_energies(J, ctx, stream)
+    log("reference energies done (largest residual %.2e)" % ref_r)
 
     def err_of(idx, ev):
         return max(abs(ev[k] - ref[idx][k]) for k in range(len(ev)))
+
+    def worst(items):
+        """Three largest errors: (request, n, electrons, roots, |dE|, final residuals)."""
+        out = sorted(items, key=lambda x: -x[1])[:3]
+        return [{"request": i, "norbs": stream[i][1]["norbs"], "nelec_a": stream[i][1]["na"],
+                 "roots": stream[i][1]["nroots"], "abs_err_Ha": e,
+                 "rnorm": [float("%.3g" % r) for r in rn]} for i, e, rn in out]
     err_m = max(err_of(r.idx, r.energies) for r in done_m)
     err_b = max(err_of(r.idx, r.energies) for r in done_b)
     err_p = max(err_of(i, ev) for i, _, ev, _ in plain)
+    worst_m = worst([(r.idx, err_of(r.idx, r.energies), r.stats["rnorm"]) for r in done_m])
+    worst_p = worst([(i, err_of(i, ev), st["rnorm"]) for i, _, ev, st in plain])
     lat_m = [r.t_done - r.arrival for r in done_m]
     lat_b = [r.t_done - r.arrival for r in done_b]
     lat_p = [lat for _, lat, _, _ in plain]
@@ -263,13 +281,19 @@ def bench_stream(J, ctx, args, rate, seed):
         "mixed": {"solves_per_s": len(done_m) / wall_m, "p50_s": percentile(lat_m, 0.5),
                   "p99_s": percentile(lat_m, 0.99), "max_abs_err_Ha": err_m,
                   "converged": sum(1 for r in done_m if r.stats["done"] == 1),
-                  "sigma_s": sig_m, "path": "served: continuous batching, mixed precision"},
+                  "sigma_s": sig_m, "step_phases_s": phases.get(J.PREC_MIXED),
+                  "worst": worst_m,
+                  "path": "served: continuous batching, mixed precision"},
         "plain_fp64": {"solves_per_s": len(plain) / wall_p, "p50_s": percentile(lat_p, 0.5),
                        "p99_s": percentile(lat_p, 0.99), "max_abs_err_Ha": err_p,
+                       "worst": worst_p,
                        "path": "one request at a time in arrival order, FP64"},
         "served_fp64": {"solves_per_s": len(done_b) / wall_b, "p50_s": percentile(lat_b, 0.5),
                         "p99_s": percentile(lat_b, 0.99), "max_abs_err_Ha": err_b,
-                        "sigma_s": sig_b, "path": "served: continuous batching, FP64"},
+                        "sigma_s": sig_b, "step_phases_s": phases.get(J.PREC_FP64),
+                        "path": "served: continuous batching, FP64"},
+        "reference_solves": {"restol": 1e-8, "largest_final_residual": ref_r,
+                             "what": "FP64 GPU solve of every request, 2 buffer roots"},
         "mixed_over_plain": wall_p / wall_m,
         "_cpu": (stream[samp][1], stream[samp][2], (len(plain), elements)),
         "_big": (stream[big][1], stream[big][2]),

@@ -281,6 +281,10 @@ int jci_batch_result(jci_batch *b, int slot, int *done, double *evals, double *e
                      jci_solve_stats *stats);
 int jci_batch_release(jci_batch *b, int slot);
 double jci_batch_sigma_seconds(const jci_batch *b);
+/* host wall seconds spent per step phase since creation: [0] sigma, [1] Gram and
+ * Rayleigh-Ritz, [2] residuals and decisions, [3] selection / collapse / precision
+ * hand-over, [4] Gram-Schmidt and orthonormalisation, [5] append */
+int jci_batch_phase_seconds(const jci_batch *b, double *out6);
 
 /* Synthetic integrals: h_ii = -2.0 + 0.25 i (1-based i), h_ij ~ N(0, 0.05^2);
  * (ij|kl) = sum_{P=1..2n} L^P_ij L^P_kl with L^P symmetric, L^P_ii ~ U(0.1, 0.3),

@@ -97,6 +97,7 @@ _sig("jci_batch_run", _i, [_P, _i])
 _sig("jci_batch_result", _i, [_P, _i, ctypes.POINTER(_i), _P, _P, ctypes.POINTER(SolveStats)])
 _sig("jci_batch_release", _i, [_P, _i])
 _sig("jci_batch_sigma_seconds", _d, [_P])
+_sig("jci_batch_phase_seconds", _i, [_P, _P])
 
 
 class JciError(RuntimeError):
@@ -476,3 +477,12 @@ class Batch:
     @property
     def sigma_seconds(self):
         return _L.jci_batch_sigma_seconds(self._h)
+
+    @property
+    def phase_seconds(self):
+        """Host seconds per step phase: sigma, Gram/Rayleigh-Ritz, residuals/decisions,
+        selection/collapse/hand-over, orthonormalisation, append."""
+        out = np.zeros(6)
+        _check(_L.jci_batch_phase_seconds(self._h, _p(out)), "jci_batch_phase_seconds")
+        return dict(zip(("sigma", "rayleigh_ritz", "residual", "collapse", "orthonormalise",
+                         "append"), out.tolist()))

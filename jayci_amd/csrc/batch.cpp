@@ -50,14 +50,29 @@ struct BInst {
     int m = 0, krymin = 0, krymax = 0, nroots = 0, ntrack = 0;
     int phase32 = 0;
     int sig_first = 0, sig_count = 0;  // basis columns whose sigma is pending
+    int kc = 0;                        // collapse target of this step (0: none)
     int iters = 0;
     long long ns64 = 0, ns32 = 0;
     std::vector<double> theta, Yf;
+    std::vector<double> G;             // projected matrix V^T S (m x m), kept across steps
+    int gnew = 0;                      // first basis column whose sigma arrived this step
     double rn[JCI_MAXROOTS] = {0};
     double ev[JCI_MAXROOTS] = {0};
     std::vector<int> todo;
     int kept = 0;
     double t_add = 0, t_done = 0;
+};
+
+// pinned staging areas for descriptor / coefficient uploads, used in rotation: an area is
+// rewritten only after the event recorded behind its last upload has completed, so the host
+// never waits for the whole stream just to refill a buffer
+#define JB_STAGES 4
+struct Stage {
+    BHeader *hH;
+    BDesc *hD;
+    double *hcoef;
+    hipEvent_t ev;
+    int used;
 };
 
 struct GraphKey {
@@ -77,6 +92,10 @@ struct jci_batch {
     BSigma *dS, *hS;
     double *dcoef, *hcoef;
     size_t coef_cap;
+    Stage stg[JB_STAGES];
+    int stg_cur;
+    int desc_cap;      // descriptors per launch (dD and every stage's hD)
+    BHeader *hHs;      // header of the current stage
     double *dout, *hout;
     size_t out_cap;
     double *dpartial;
@@ -90,6 +109,10 @@ struct jci_batch {
     size_t epool_bytes, epool_want;
     long long steps;
     double sigma_seconds;
+    // host wall time per step phase: 0 sigma (all of run_sigma), 1 Gram + Rayleigh-Ritz,
+    // 2 residuals and decisions, 3 selection / collapse / hand-over, 4 Gram-Schmidt and
+    // orthonormalisation, 5 append
+    double phase_seconds[6];
 };
 
 static int nblk_rows(long long N) { return (int)((N + 255) / 256); }
@@ -113,13 +136,25 @@ JCI_API int jci_batch_create(jci_ctx *c, int capacity, int max_norbs, jci_batch 
     b->use_graphs = 1;
     JCI_CHECK(hipMalloc(&b->dH, sizeof(BHeader) * 8));
     JCI_CHECK(hipHostMalloc(&b->hH, sizeof(BHeader) * 8, 0));
-    JCI_CHECK(hipMalloc(&b->dD, sizeof(BDesc) * capacity));
-    JCI_CHECK(hipHostMalloc(&b->hD, sizeof(BDesc) * capacity, 0));
+    // descriptors per launch: at most two per instance (V and S rotated together)
+    b->desc_cap = 2 * capacity + 8;
+    JCI_CHECK(hipMalloc(&b->dD, sizeof(BDesc) * b->desc_cap));
     JCI_CHECK(hipMalloc(&b->dS, sizeof(BSigma) * capacity * 2));
     JCI_CHECK(hipHostMalloc(&b->hS, sizeof(BSigma) * capacity * 2, 0));
     b->coef_cap = (size_t)capacity * (JCI_MAXBASIS * JCI_MAXBASIS + JCI_MAXK);
     JCI_CHECK(hipMalloc(&b->dcoef, sizeof(double) * b->coef_cap));
-    JCI_CHECK(hipHostMalloc(&b->hcoef, sizeof(double) * b->coef_cap, 0));
+    for (int i = 0; i < JB_STAGES; i++) {
+        Stage &g = b->stg[i];
+        JCI_CHECK(hipHostMalloc(&g.hH, sizeof(BHeader), 0));
+        JCI_CHECK(hipHostMalloc(&g.hD, sizeof(BDesc) * b->desc_cap, 0));
+        JCI_CHECK(hipHostMalloc(&g.hcoef, sizeof(double) * b->coef_cap, 0));
+        JCI_CHECK(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
+        g.used = 0;
+    }
+    b->stg_cur = 0;
+    b->hD = b->stg[0].hD;
+    b->hcoef = b->stg[0].hcoef;
+    b->hHs = b->stg[0].hH;
     b->out_cap = (size_t)capacity * JCI_MAXBASIS * JCI_MAXBASIS;
     JCI_CHECK(hipMalloc(&b->dout, sizeof(double) * b->out_cap));
     JCI_CHECK(hipHostMalloc(&b->hout, sizeof(double) * b->out_cap, 0));
@@ -151,9 +186,13 @@ JCI_API void jci_batch_destroy(jci_batch *b)
     for (auto &s : b->in) if (s.used) free_inst(s);
     for (auto &g : b->graphs) hipGraphExecDestroy(g.exec);
     hipFree(b->dH); hipHostFree(b->hH);
-    hipFree(b->dD); hipHostFree(b->hD);
+    hipFree(b->dD);
+    for (int i = 0; i < JB_STAGES; i++) {
+        hipHostFree(b->stg[i].hH); hipHostFree(b->stg[i].hD); hipHostFree(b->stg[i].hcoef);
+        hipEventDestroy(b->stg[i].ev);
+    }
     hipFree(b->dS); hipHostFree(b->hS);
-    hipFree(b->dcoef); hipHostFree(b->hcoef);
+    hipFree(b->dcoef);
     hipFree(b->dout); hipHostFree(b->hout);
     hipFree(b->dpartial);
     hipFree(b->dGH); hipHostFree(b->hGH);
@@ -245,11 +284,35 @@ JCI_API int jci_batch_release(jci_batch *b, int slot)
 }
 
 // ---- helpers ------------------------------------------------------------------------------
+// move to the next staging area (waits only for that area's last uploads); b->hD, b->hcoef
+// then point into it
+static int stage_next(jci_batch *b)
+{
+    b->stg_cur = (b->stg_cur + 1) % JB_STAGES;
+    Stage &g = b->stg[b->stg_cur];
+    if (g.used) JCI_CHECK(hipEventSynchronize(g.ev));
+    g.used = 0;
+    b->hD = g.hD;
+    b->hcoef = g.hcoef;
+    b->hHs = g.hH;
+    return 0;
+}
+
+// mark the current area busy until the uploads queued so far have run
+static int stage_done(jci_batch *b)
+{
+    Stage &g = b->stg[b->stg_cur];
+    JCI_CHECK(hipEventRecord(g.ev, b->c->stream));
+    g.used = 1;
+    return 0;
+}
+
 static int upload_desc(jci_batch *b, int n, long long total, int hslot = 0)
 {
-    b->hH[hslot].n = n;
-    b->hH[hslot].total = total;
-    JCI_CHECK(hipMemcpyAsync(b->dH + hslot, b->hH + hslot, sizeof(BHeader), hipMemcpyHostToDevice,
+    if (n > b->desc_cap) { jci_set_error("batch: %d descriptors above %d", n, b->desc_cap); return -1; }
+    b->hHs->n = n;
+    b->hHs->total = total;
+    JCI_CHECK(hipMemcpyAsync(b->dH + hslot, b->hHs, sizeof(BHeader), hipMemcpyHostToDevice,
                              b->c->stream));
     if (n > 0)
         JCI_CHECK(hipMemcpyAsync(b->dD, b->hD, sizeof(BDesc) * n, hipMemcpyHostToDevice, b->c->stream));
@@ -276,9 +339,10 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
                     std::vector<double *> B, std::vector<int> m, std::vector<int> w,
                     std::vector<size_t> &ofs)
 {
-    // the pinned staging buffers are rewritten below: earlier copies must have drained
-    JCI_CHECK(hipStreamSynchronize(b->c->stream));
     int n = (int)ids.size();
+    if (n == 0) { ofs.clear(); return 0; }
+    if (n > b->desc_cap) { jci_set_error("batch: %d descriptors above %d", n, b->desc_cap); return -1; }
+    RCHECK(stage_next(b));
     long long blk = 0, pofs = 0;
     size_t o = 0;
     ofs.assign(n, 0);
@@ -296,12 +360,14 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
     }
     if ((size_t)pofs > b->partial_cap || o > b->out_cap) { jci_set_error("batch: scratch too small"); return -1; }
     RCHECK(upload_desc(b, n, blk));
+    RCHECK(stage_done(b));
     RCHECK(jb_gram(b->dH, b->dD, n, b->dpartial, b->c->stream));
     RCHECK(download_out(b, o));
     return 0;
 }
 
-// O_t = beta O_t + A_t C_t for the listed instances (coefficients already in hcoef at cofs)
+// O_t = beta O_t + A_t C_t for the listed instances; the caller has moved to a fresh staging
+// area (stage_next) and written the coefficients into hcoef at cofs
 static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vector<double *> &A,
                        const std::vector<double *> &O, const std::vector<int> &m,
                        const std::vector<int> &k, const std::vector<size_t> &cofs,
@@ -309,7 +375,7 @@ static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vec
 {
     int n = (int)ids.size();
     if (n == 0) return 0;
-    JCI_CHECK(hipStreamSynchronize(b->c->stream));
+    if (n > b->desc_cap) { jci_set_error("batch: %d descriptors above %d", n, b->desc_cap); return -1; }
     long long blk = 0;
     for (int t = 0; t < n; t++) {
         BInst &s = b->in[ids[t]];
@@ -322,6 +388,7 @@ static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vec
     }
     RCHECK(upload_coef(b, coef_len));
     RCHECK(upload_desc(b, n, blk));
+    RCHECK(stage_done(b));
     RCHECK(jb_combine(b->dH, b->dD, b->c->stream));
     return 0;
 }
@@ -549,6 +616,15 @@ static int run_sigma_tab(jci_batch *b, const std::vector<int> &ids, int prec)
 }
 
 // ---- one lockstep iteration -----------------------------------------------------------------
+// Per step: (1) sigma of every pending basis column (grouped); (2) the new columns of the
+// projected matrix G = V^T S (one batched Gram; the rest of G is kept) and a host eigensolve
+// per instance; (3) residuals of the tracked Ritz pairs with the diagonal preconditioner;
+// (4) decisions (precision hand-over, convergence, collapse); (5) collapse / hand-over
+// rotations; (6) the selected corrections appended to V; (7) one Gram [V_old | V_new]^T V_new
+// gives both the projection C = V_old^T V_new and the block's own overlap; the projected
+// overlap is V_new^T V_new - C^T C, and one combine writes (V_new - V_old C) L in place, L its
+// inverse square root on the kept directions.  Where the projection cancels most of a
+// vector, an explicit projection pass runs first (Gram-Schmidt twice).
 JCI_API int jci_batch_step(jci_batch *b, int *nactive)
 {
     jci_ctx *c = b->c;
@@ -559,39 +635,61 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
     *nactive = (int)act.size();
     if (act.empty()) return 0;
     b->steps++;
+    double tp = tnow();
     // 1. pending sigma (FP32 and FP64 groups)
     std::vector<int> g32, g64;
     for (int i : act) {
         BInst &s = b->in[i];
+        s.gnew = s.sig_count > 0 ? s.sig_first : s.m;
         if (s.sig_count > 0) (s.phase32 ? g32 : g64).push_back(i);
     }
     RCHECK(run_sigma(b, g32, 32));
     RCHECK(run_sigma(b, g64, 64));
-    // 2. Gram V^T S and Rayleigh-Ritz
-    std::vector<size_t> ofs;
+    b->phase_seconds[0] += tnow() - tp;
+    tp = tnow();
+    // 2. new columns of G and Rayleigh-Ritz
     {
+        std::vector<int> ids;
         std::vector<double *> A, B;
         std::vector<int> m, w;
+        std::vector<size_t> ofs;
         for (int i : act) {
             BInst &s = b->in[i];
-            A.push_back(s.V); B.push_back(s.S); m.push_back(s.m); w.push_back(s.m);
+            if (s.G.size() != (size_t)s.gnew * s.gnew) s.gnew = 0;   // no valid leading block
+            if (s.gnew >= s.m) continue;
+            ids.push_back(i); A.push_back(s.V); B.push_back(s.S + (size_t)s.gnew * s.ld);
+            m.push_back(s.m); w.push_back(s.m - s.gnew);
         }
-        RCHECK(run_gram(b, act, A, B, m, w, ofs));
+        RCHECK(run_gram(b, ids, A, B, m, w, ofs));
+        for (size_t t = 0; t < ids.size(); t++) {
+            BInst &s = b->in[ids[t]];
+            const int mm = s.m, g0 = s.gnew, ww = mm - g0;
+            std::vector<double> G((size_t)mm * mm, 0.0);
+            for (int i = 0; i < g0; i++)
+                for (int j = 0; j < g0; j++) G[(size_t)i * mm + j] = s.G[(size_t)i * g0 + j];
+            const double *o = b->hout + ofs[t];
+            for (int i = 0; i < mm; i++)
+                for (int j = g0; j < mm; j++) {
+                    double v = o[(size_t)i * ww + (j - g0)];
+                    if (i >= g0) v = 0.5 * (v + o[(size_t)j * ww + (i - g0)]);
+                    G[(size_t)i * mm + j] = G[(size_t)j * mm + i] = v;
+                }
+            s.G.swap(G);
+        }
     }
-    for (size_t t = 0; t < act.size(); t++) {
-        BInst &s = b->in[act[t]];
-        int m = s.m;
-        std::vector<double> G(b->hout + ofs[t], b->hout + ofs[t] + (size_t)m * m);
-        for (int i = 0; i < m; i++)
-            for (int j = 0; j < i; j++) G[i * m + j] = G[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
-        s.theta.assign(m, 0.0);
-        s.Yf.assign((size_t)m * m, 0.0);
-        jci_jacobi_eig(m, G.data(), s.theta.data(), s.Yf.data());
+    for (int i : act) {
+        BInst &s = b->in[i];
+        s.theta.assign(s.m, 0.0);
+        s.Yf.assign((size_t)s.m * s.m, 0.0);
+        jci_jacobi_eig(s.m, s.G.data(), s.theta.data(), s.Yf.data());
         s.iters++;
     }
+    b->phase_seconds[1] += tnow() - tp;
+    tp = tnow();
     // 3. residuals / corrections -> T, norms
-    JCI_CHECK(hipStreamSynchronize(c->stream));
+    std::vector<size_t> ofs(act.size());
     {
+        RCHECK(stage_next(b));
         size_t co = 0, o = 0;
         long long blk = 0, pofs = 0;
         for (size_t t = 0; t < act.size(); t++) {
@@ -613,12 +711,14 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
         }
         RCHECK(upload_coef(b, co));
         RCHECK(upload_desc(b, (int)act.size(), blk));
+        RCHECK(stage_done(b));
         RCHECK(jb_residual(b->dH, b->dD, (int)act.size(), b->dpartial, c->stream));
         RCHECK(download_out(b, o));
     }
     // 4. decisions per instance
     std::vector<int> grow;          // instances adding vectors this step
     std::vector<int> handover;      // FP32 -> FP64: collapse onto Ritz vectors, re-multiply
+    std::vector<int> collapse;      // full subspace: collapse onto kc Ritz vectors
     for (size_t t = 0; t < act.size(); t++) {
         BInst &s = b->in[act[t]];
         const int nt = std::min(s.ntrack, s.m);
@@ -645,44 +745,67 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             if (!(rall[r] < s.p.restol)) s.todo.push_back(r);
         // a full subspace collapses onto kc Ritz vectors; buffer corrections that do not fit
         // next to them are dropped
+        s.kc = 0;
         if (s.m + (int)s.todo.size() > s.krymax) {
             int kc = std::max(s.krymin, nt);
             kc = std::min(kc, s.krymax - (int)s.todo.size());
             if (kc < s.nroots) kc = s.nroots;
             if (kc + (int)s.todo.size() > s.krymax) s.todo.resize(s.krymax - kc);
+            s.kc = kc;
+            collapse.push_back(act[t]);
         }
         grow.push_back(act[t]);
     }
-    // hand-over: V <- V Y[:, :kc] (the lowest kc Ritz vectors of the FP32 phase); their FP64
-    // sigma is pending for the next step
-    if (!handover.empty()) {
-        JCI_CHECK(hipStreamSynchronize(c->stream));
+    b->phase_seconds[2] += tnow() - tp;
+    tp = tnow();
+    // 5. rotations: hand-over (V onto its lowest kc Ritz vectors; their FP64 sigma is pending
+    // for the next step) and collapse (V and S onto kc Ritz vectors, G becomes diagonal)
+    {
+        RCHECK(stage_next(b));
         std::vector<int> ids;
         std::vector<double *> A, O;
         std::vector<int> m, k;
         std::vector<size_t> cofs;
         size_t co = 0;
-        for (int i : handover) {
-            BInst &s = b->in[i];
-            int kc = std::min(s.m, s.krymin);
+        auto add_rot = [&](BInst &s, int i, int kc, bool with_s) {
             for (int r = 0; r < s.m; r++)
                 for (int j = 0; j < kc; j++) b->hcoef[co + (size_t)r * kc + j] = s.Yf[(size_t)j * s.m + r];
             ids.push_back(i); A.push_back(s.V); O.push_back(s.V); m.push_back(s.m); k.push_back(kc);
             cofs.push_back(co);
+            if (with_s) {
+                ids.push_back(i); A.push_back(s.S); O.push_back(s.S); m.push_back(s.m); k.push_back(kc);
+                cofs.push_back(co);
+            }
             co += (size_t)s.m * kc;
+        };
+        for (int i : handover) {
+            BInst &s = b->in[i];
+            add_rot(s, i, std::min(s.m, s.krymin), false);
+        }
+        for (int i : collapse) {
+            BInst &s = b->in[i];
+            add_rot(s, i, s.kc, true);
         }
         RCHECK(run_combine(b, ids, A, O, m, k, cofs, 0.0, co));
-        for (size_t t = 0; t < ids.size(); t++) {
-            BInst &s = b->in[ids[t]];
-            s.m = k[t];
+        for (int i : handover) {
+            BInst &s = b->in[i];
+            s.m = std::min(s.m, s.krymin);
+            s.G.clear();
             s.sig_first = 0;
             s.sig_count = s.m;
         }
+        for (int i : collapse) {
+            BInst &s = b->in[i];
+            const int kc = s.kc;
+            s.G.assign((size_t)kc * kc, 0.0);
+            for (int j = 0; j < kc; j++) s.G[(size_t)j * kc + j] = s.theta[j];
+            s.m = kc;
+        }
     }
-    if (grow.empty()) return 0;
-    // 5. select unconverged corrections (in place) and collapse full subspaces
-    JCI_CHECK(hipStreamSynchronize(c->stream));
+    if (grow.empty()) { b->phase_seconds[3] += tnow() - tp; return 0; }
+    // 6. append the selected corrections: V[m .. m+nk) = T Sel
     {
+        RCHECK(stage_next(b));
         std::vector<int> ids;
         std::vector<double *> A, O;
         std::vector<int> m, k;
@@ -690,127 +813,119 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
         size_t co = 0;
         for (int i : grow) {
             BInst &s = b->in[i];
-            int nk = (int)s.todo.size();
-            const int nt = std::min(s.ntrack, s.m);
-            bool ident = nk == nt;
-            for (int j = 0; j < nk && ident; j++) ident = s.todo[j] == j;
-            if (ident) continue;
+            const int nt = std::min(s.ntrack, (int)s.theta.size());
+            const int nk = (int)s.todo.size();
             for (int r = 0; r < nt; r++)
                 for (int j = 0; j < nk; j++) b->hcoef[co + (size_t)r * nk + j] = (s.todo[j] == r) ? 1.0 : 0.0;
-            ids.push_back(i); A.push_back(s.T); O.push_back(s.T); m.push_back(nt); k.push_back(nk);
-            cofs.push_back(co);
+            ids.push_back(i); A.push_back(s.T); O.push_back(s.V + (size_t)s.m * s.ld); m.push_back(nt);
+            k.push_back(nk); cofs.push_back(co);
             co += (size_t)nt * nk;
         }
         RCHECK(run_combine(b, ids, A, O, m, k, cofs, 0.0, co));
     }
-    JCI_CHECK(hipStreamSynchronize(c->stream));
-    {
-        std::vector<int> ids;
-        std::vector<double *> A, O;
-        std::vector<int> m, k;
-        std::vector<size_t> cofs;
-        size_t co = 0;
-        std::vector<int> newm;
-        for (int i : grow) {
-            BInst &s = b->in[i];
-            int nk = (int)s.todo.size();
-            if (s.m + nk <= s.krymax) continue;
-            int kc = std::max(s.krymin, std::min(s.ntrack, s.m));
-            kc = std::min(kc, s.krymax - nk);
-            if (kc < s.nroots) kc = s.nroots;
-            for (int r = 0; r < s.m; r++)
-                for (int j = 0; j < kc; j++) b->hcoef[co + (size_t)r * kc + j] = s.Yf[(size_t)j * s.m + r];
-            // rotate V and S with the same coefficients
-            ids.push_back(i); A.push_back(s.V); O.push_back(s.V); m.push_back(s.m); k.push_back(kc); cofs.push_back(co);
-            ids.push_back(i); A.push_back(s.S); O.push_back(s.S); m.push_back(s.m); k.push_back(kc); cofs.push_back(co);
-            co += (size_t)s.m * kc;
-            newm.push_back(kc);
-        }
-        RCHECK(run_combine(b, ids, A, O, m, k, cofs, 0.0, co));
-        for (size_t t = 0, u = 0; t < ids.size(); t += 2, u++) b->in[ids[t]].m = newm[u];
-    }
-    // 6. two classical Gram-Schmidt passes of T against V
-    for (int pass = 0; pass < 2; pass++) {
-        std::vector<int> ids;
+    b->phase_seconds[3] += tnow() - tp;
+    tp = tnow();
+    // 7. orthonormalise the new blocks against the basis and among themselves
+    std::vector<int> pend = grow, fin;
+    std::vector<std::vector<double>> W(b->cap);   // final coefficients per slot
+    for (int pass = 0; pass < 2 && !pend.empty(); pass++) {
         std::vector<double *> A, B;
         std::vector<int> m, w;
-        for (int i : grow) {
+        std::vector<size_t> gofs;
+        for (int i : pend) {
             BInst &s = b->in[i];
-            ids.push_back(i); A.push_back(s.V); B.push_back(s.T); m.push_back(s.m);
-            w.push_back((int)s.todo.size());
+            const int nk = (int)s.todo.size();
+            A.push_back(s.V); B.push_back(s.V + (size_t)s.m * s.ld);
+            m.push_back(s.m + nk); w.push_back(nk);
         }
-        RCHECK(run_gram(b, ids, A, B, m, w, ofs));
-        std::vector<double *> AA, O;
-        std::vector<int> mm, kk;
-        std::vector<size_t> cofs;
-        size_t co = 0;
-        JCI_CHECK(hipStreamSynchronize(c->stream));
-        for (size_t t = 0; t < ids.size(); t++) {
-            BInst &s = b->in[ids[t]];
-            int nk = (int)s.todo.size();
-            for (int e = 0; e < s.m * nk; e++) b->hcoef[co + e] = -b->hout[ofs[t] + e];
-            AA.push_back(s.V); O.push_back(s.T); mm.push_back(s.m); kk.push_back(nk); cofs.push_back(co);
-            co += (size_t)s.m * nk;
-        }
-        RCHECK(run_combine(b, ids, AA, O, mm, kk, cofs, 1.0, co));
-    }
-    // 7. symmetric orthonormalisation inside T
-    {
-        std::vector<double *> A, B;
-        std::vector<int> m, w;
-        for (int i : grow) {
-            BInst &s = b->in[i];
-            int nk = (int)s.todo.size();
-            A.push_back(s.T); B.push_back(s.T); m.push_back(nk); w.push_back(nk);
-        }
-        RCHECK(run_gram(b, grow, A, B, m, w, ofs));
-        std::vector<int> ids;
-        std::vector<double *> AA, O;
-        std::vector<int> mm, kk;
-        std::vector<size_t> cofs;
-        size_t co = 0;
-        for (size_t t = 0; t < grow.size(); t++) {
-            BInst &s = b->in[grow[t]];
-            int k = (int)s.todo.size();
-            std::vector<double> g(b->hout + ofs[t], b->hout + ofs[t] + (size_t)k * k), ev(k), U((size_t)k * k);
-            for (int i = 0; i < k; i++)
-                for (int j = 0; j < i; j++) g[i * k + j] = g[j * k + i] = 0.5 * (g[i * k + j] + g[j * k + i]);
-            jci_jacobi_eig(k, g.data(), ev.data(), U.data());
-            double emax = 0.0;
+        RCHECK(run_gram(b, pend, A, B, m, w, gofs));
+        std::vector<int> redo;
+        for (size_t t = 0; t < pend.size(); t++) {
+            BInst &s = b->in[pend[t]];
+            const int mo = s.m, nk = (int)s.todo.size();
+            const double *g = b->hout + gofs[t];           // (mo + nk) x nk
+            // C = V_old^T V_new (mo x nk), TT = V_new^T V_new, TTp = TT - C^T C
+            std::vector<double> TT((size_t)nk * nk), TTp((size_t)nk * nk);
+            for (int i = 0; i < nk; i++)
+                for (int j = 0; j < nk; j++) {
+                    double v = 0.5 * (g[(size_t)(mo + i) * nk + j] + g[(size_t)(mo + j) * nk + i]);
+                    double cc = 0.0;
+                    for (int r = 0; r < mo; r++) cc += g[(size_t)r * nk + i] * g[(size_t)r * nk + j];
+                    TT[(size_t)i * nk + j] = v;
+                    TTp[(size_t)i * nk + j] = v - cc;
+                }
+            std::vector<double> ev(nk), U((size_t)nk * nk);
+            jci_jacobi_eig(nk, TTp.data(), ev.data(), U.data());
+            double emax = 0.0, cancel = 1.0;
             for (double e : ev) emax = std::max(emax, e);
-            int nk = 0;
-            for (int i = k - 1; i >= 0; i--) if (ev[i] > 1e-12 * emax && ev[i] > 1e-24) nk++;
-            s.kept = nk;
-            if (nk == 0) continue;
+            for (int j = 0; j < nk; j++)
+                if (TT[(size_t)j * nk + j] > 0) cancel = std::min(cancel, TTp[(size_t)j * nk + j] / TT[(size_t)j * nk + j]);
+            const bool weak = cancel < 0.5 || ev[0] < 1e-6 * emax;
+            if (pass == 0 && weak) { redo.push_back(pend[t]); continue; }
+            // kept directions (largest first) and W = [-C L; L]
+            int nkk = 0;
+            for (int i = nk - 1; i >= 0; i--) if (ev[i] > 1e-12 * emax && ev[i] > 1e-24) nkk++;
+            s.kept = nkk;
+            if (nkk == 0) { fin.push_back(pend[t]); continue; }
+            std::vector<double> L((size_t)nk * nkk);
             int col = 0;
-            for (int i = k - 1; i >= 0 && col < nk; i--) {
+            for (int i = nk - 1; i >= 0 && col < nkk; i--) {
                 if (!(ev[i] > 1e-12 * emax && ev[i] > 1e-24)) continue;
-                double sc = 1.0 / sqrt(ev[i]);
-                for (int r = 0; r < k; r++) b->hcoef[co + (size_t)r * nk + col] = U[(size_t)i * k + r] * sc;
+                const double sc = 1.0 / sqrt(ev[i]);
+                for (int r = 0; r < nk; r++) L[(size_t)r * nkk + col] = U[(size_t)i * nk + r] * sc;
                 col++;
             }
-            ids.push_back(grow[t]); AA.push_back(s.T); O.push_back(s.T); mm.push_back(k); kk.push_back(nk);
-            cofs.push_back(co);
-            co += (size_t)k * nk;
+            std::vector<double> &Wt = W[pend[t]];
+            Wt.assign((size_t)(mo + nk) * nkk, 0.0);
+            for (int r = 0; r < mo; r++)
+                for (int j = 0; j < nkk; j++) {
+                    double v = 0.0;
+                    for (int q = 0; q < nk; q++) v += g[(size_t)r * nk + q] * L[(size_t)q * nkk + j];
+                    Wt[(size_t)r * nkk + j] = -v;
+                }
+            for (int r = 0; r < nk; r++)
+                for (int j = 0; j < nkk; j++) Wt[(size_t)(mo + r) * nkk + j] = L[(size_t)r * nkk + j];
+            fin.push_back(pend[t]);
         }
-        RCHECK(run_combine(b, ids, AA, O, mm, kk, cofs, 0.0, co));
+        if (!redo.empty()) {
+            // explicit projection V_new -= V_old C, then the Gram again
+            RCHECK(stage_next(b));
+            std::vector<double *> AA, O;
+            std::vector<int> mm, kk;
+            std::vector<size_t> cofs;
+            size_t co = 0;
+            for (size_t t = 0; t < pend.size(); t++) {
+                if (std::find(redo.begin(), redo.end(), pend[t]) == redo.end()) continue;
+                BInst &s = b->in[pend[t]];
+                const int mo = s.m, nk = (int)s.todo.size();
+                const double *g = b->hout + gofs[t];
+                for (int e = 0; e < mo * nk; e++) b->hcoef[co + e] = -g[e];
+                AA.push_back(s.V); O.push_back(s.V + (size_t)mo * s.ld); mm.push_back(mo); kk.push_back(nk);
+                cofs.push_back(co);
+                co += (size_t)mo * nk;
+            }
+            RCHECK(run_combine(b, redo, AA, O, mm, kk, cofs, 1.0, co));
+        }
+        pend.swap(redo);
     }
-    // 8. append T to V: V[m .. m+kept) = T (identity coefficients)
-    JCI_CHECK(hipStreamSynchronize(c->stream));
+    // final combine: V[m .. m+kept) = V[0 .. m+nk) W (in place: each row is read before it is
+    // written)
     {
+        RCHECK(stage_next(b));
         std::vector<int> ids;
         std::vector<double *> A, O;
         std::vector<int> m, k;
         std::vector<size_t> cofs;
         size_t co = 0;
-        for (int i : grow) {
+        for (int i : fin) {
             BInst &s = b->in[i];
             if (s.kept == 0) { s.done = 1; s.failed = 1; s.t_done = tnow(); continue; }
-            for (int r = 0; r < s.kept; r++)
-                for (int j = 0; j < s.kept; j++) b->hcoef[co + (size_t)r * s.kept + j] = (r == j) ? 1.0 : 0.0;
-            ids.push_back(i); A.push_back(s.T); O.push_back(s.V + (size_t)s.m * s.ld); m.push_back(s.kept);
-            k.push_back(s.kept); cofs.push_back(co);
-            co += (size_t)s.kept * s.kept;
+            const int nk = (int)s.todo.size();
+            const std::vector<double> &Wt = W[i];
+            memcpy(b->hcoef + co, Wt.data(), sizeof(double) * Wt.size());
+            ids.push_back(i); A.push_back(s.V); O.push_back(s.V + (size_t)s.m * s.ld);
+            m.push_back(s.m + nk); k.push_back(s.kept); cofs.push_back(co);
+            co += Wt.size();
         }
         RCHECK(run_combine(b, ids, A, O, m, k, cofs, 0.0, co));
         for (int i : ids) {
@@ -820,7 +935,7 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             s.m += s.kept;
         }
     }
-    JCI_CHECK(hipStreamSynchronize(c->stream));
+    b->phase_seconds[4] += tnow() - tp;
     return 0;
 }
 
@@ -870,3 +985,9 @@ JCI_API int jci_batch_run(jci_batch *b, int max_steps)
 }
 
 JCI_API double jci_batch_sigma_seconds(const jci_batch *b) { return b->sigma_seconds; }
+
+JCI_API int jci_batch_phase_seconds(const jci_batch *b, double *out6)
+{
+    for (int i = 0; i < 6; i++) out6[i] = b->phase_seconds[i];
+    return 0;
+}

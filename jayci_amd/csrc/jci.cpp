@@ -778,57 +778,12 @@ JCI_API int jci_sigma(jci_ham *h, int method, int nvec, const double *x, double 
 
 // ---- small dense eigensolver (cyclic Jacobi, FP64) ------------------------------------------
 // eigenvalues ascending; eigenvector i in vec[i*n + 0..n-1]
+void jci_sym_eig(int n, const double *a, double *eval, double *vec);   // symeig.cpp
+
+// subspace eigensolver (eigenvalues ascending, vectors row-major with a fixed sign)
 static void jacobi_eig(int n, const double *a_in, double *eval, double *vec)
 {
-    std::vector<double> a(a_in, a_in + (size_t)n * n), v((size_t)n * n, 0.0);
-    for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
-    for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0.0, tot = 0.0;
-        for (int i = 0; i < n; i++)
-            for (int j = 0; j < n; j++) {
-                double x = a[(size_t)i * n + j] * a[(size_t)i * n + j];
-                tot += x;
-                if (i != j) off += x;
-            }
-        if (off <= 1e-30 * (tot > 0 ? tot : 1.0)) break;
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                double apq = a[(size_t)p * n + q];
-                if (fabs(apq) < 1e-300) continue;
-                double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2.0 * apq);
-                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
-                for (int k = 0; k < n; k++) {
-                    double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-                    a[(size_t)k * n + p] = cs * akp - sn * akq;
-                    a[(size_t)k * n + q] = sn * akp + cs * akq;
-                }
-                for (int k = 0; k < n; k++) {
-                    double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-                    a[(size_t)p * n + k] = cs * apk - sn * aqk;
-                    a[(size_t)q * n + k] = sn * apk + cs * aqk;
-                }
-                for (int k = 0; k < n; k++) {
-                    double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
-                    v[(size_t)k * n + p] = cs * vkp - sn * vkq;
-                    v[(size_t)k * n + q] = sn * vkp + cs * vkq;
-                }
-            }
-    }
-    std::vector<int> ord(n);
-    for (int i = 0; i < n; i++) ord[i] = i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
-        return a[(size_t)x * n + x] < a[(size_t)y * n + y];
-    });
-    for (int i = 0; i < n; i++) {
-        eval[i] = a[(size_t)ord[i] * n + ord[i]];
-        // fix the sign: largest-magnitude component positive (deterministic vectors)
-        int arg = 0;
-        for (int k = 1; k < n; k++)
-            if (fabs(v[(size_t)k * n + ord[i]]) > fabs(v[(size_t)arg * n + ord[i]])) arg = k;
-        double sg = v[(size_t)arg * n + ord[i]] < 0 ? -1.0 : 1.0;
-        for (int k = 0; k < n; k++) vec[(size_t)i * n + k] = sg * v[(size_t)k * n + ord[i]];
-    }
+    jci_sym_eig(n, a_in, eval, vec);
 }
 
 JCI_API int jci_gram(jci_ctx *c, int m, int w, long long n, const double *V, const double *W,

@@ -160,42 +160,61 @@ __global__ __launch_bounds__(256) void k_gcv(const GfHeader *__restrict__ H,
     }
 }
 
-// C(64x64 tile at m0, n0) += A B over Kr, A row-major (lda), B row-major (ldb), M x N bounds
+// C(64x64 tile at m0, n0) += A B over Kr, A row-major (lda), B row-major (ldb), M x N bounds.
+// Two LDS buffers: the global loads of step k0 + 16 are issued before the products of step
+// k0 and stored into the other buffer after them, one barrier per step.
 template <typename T>
 __device__ __forceinline__ void mma_tile(typename Mf<T>::acc (&acc)[2][2], const T *__restrict__ A,
                                          long long lda, const T *__restrict__ B, long long ldb,
                                          long long M, long long N, long long Kr, long long m0,
-                                         long long n0, T (*As)[65], T (*Bs)[65])
+                                         long long n0, T (*As)[16][65], T (*Bs)[16][65])
 {
     const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
     const int ar = tid >> 2, ak = (tid & 3) * 4;
     const int bk = tid >> 4, bc = (tid & 15) * 4;
-    for (long long k0 = 0; k0 < Kr; k0 += 16) {
-        const long long gm = m0 + ar;
+    const long long gm = m0 + ar;
+    T ra[4], rb[4];
+    auto gload = [&](long long k0) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const long long gk = k0 + ak + j;
-            As[ak + j][ar] = (gm < M && gk < Kr) ? A[gm * lda + gk] : (T)0;
+            ra[j] = (gm < M && gk < Kr) ? A[gm * lda + gk] : (T)0;
         }
         const long long gk = k0 + bk;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const long long gn = n0 + bc + j;
-            Bs[bk][bc + j] = (gk < Kr && gn < N) ? B[gk * ldb + gn] : (T)0;
+            rb[j] = (gk < Kr && gn < N) ? B[gk * ldb + gn] : (T)0;
         }
-        __syncthreads();
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            As[buf][ak + j][ar] = ra[j];
+            Bs[buf][bk][bc + j] = rb[j];
+        }
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (long long k0 = 0; k0 < Kr; k0 += 16) {
+        const bool more = k0 + 16 < Kr;
+        if (more) gload(k0 + 16);
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
             const int kr = ks * 4 + l / 16;
-            const T a0 = As[kr][wm * 32 + l % 16], a1 = As[kr][wm * 32 + 16 + l % 16];
-            const T b0 = Bs[kr][wn * 32 + l % 16], b1 = Bs[kr][wn * 32 + 16 + l % 16];
+            const T a0 = As[buf][kr][wm * 32 + l % 16], a1 = As[buf][kr][wm * 32 + 16 + l % 16];
+            const T b0 = Bs[buf][kr][wn * 32 + l % 16], b1 = Bs[buf][kr][wn * 32 + 16 + l % 16];
             acc[0][0] = Mf<T>::mma(a0, b0, acc[0][0]);
             acc[0][1] = Mf<T>::mma(a0, b1, acc[0][1]);
             acc[1][0] = Mf<T>::mma(a1, b0, acc[1][0]);
             acc[1][1] = Mf<T>::mma(a1, b1, acc[1][1]);
         }
+        if (more) sstore(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
 }
 
@@ -204,8 +223,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gss(const GfHeader *__restrict__ H,
                                              const GfTask *__restrict__ tasks)
 {
-    __shared__ T As[16][65];
-    __shared__ T Bs[16][65];
+    __shared__ T As[2][16][65];
+    __shared__ T Bs[2][16][65];
     const int n = H->n;
     const long long total = H->total[GF_SS];
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -244,12 +263,16 @@ __global__ __launch_bounds__(256) void k_gss(const GfHeader *__restrict__ H,
 // E[P][c] = sum_Q (P|Q) D[Q][c] over columns c = (k, Ja, Ib): the tile of D is built in LDS
 // from the beta lists (each (Q, column) receives at most one entry) and multiplied by the
 // (P|Q) fragments that every wave keeps in registers (wave w: row tiles w and w + 4),
-// reloaded when the task changes.
+// reloaded when the task changes.  Each work-group takes a contiguous run of column tiles,
+// so consecutive tiles share the vector row X_k[Ja]: it is staged in LDS once (rows of up to
+// GF_XROW strings) and the beta-list gathers read it there.
+#define GF_XROW 4096
 template <typename T, int BN>
 __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
                                             const GfTask *__restrict__ tasks)
 {
     __shared__ T Ds[GF_MAXMP][BN + 1];
+    __shared__ T xs[GF_XROW];
     constexpr int NCT = BN / 16, PARTS = 256 / BN, KSMAX = GF_MAXMP / 4;
     const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
     const int n = H->n;
@@ -257,7 +280,9 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
     const int col = tid % BN, part = tid / BN;
     T a[2][KSMAX];
     int cur = -1;
-    for (long long u = blockIdx.x; u < total; u += gridDim.x) {
+    long long staged = -1;        // (task, k, Ja) of the row in xs, as task * 2^40 + k * N + Ja*Nb
+    const long long u0 = total * blockIdx.x / gridDim.x, u1 = total * (blockIdx.x + 1) / gridDim.x;
+    for (long long u = u0; u < u1; u++) {
         const int ti = find_task(tasks, n, u, GF_E);
         const GfTask &t = tasks[ti];
         const GemmFci &g = t.g;
@@ -277,18 +302,38 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
         }
         const long long Nb = g.nb, N = g.na * g.nb, ncols = N * t.K;
         const long long c0 = (u - t.base[GF_E]) * BN;
+        // one vector row for the whole tile: stage it
+        const long long clast = std::min(c0 + BN, ncols) - 1;
+        const bool one_row = Nb <= GF_XROW && (c0 / Nb) == (clast / Nb);
+        if (one_row) {
+            const long long key = ((long long)ti << 40) + (c0 / Nb) * Nb;
+            if (key != staged) {
+                const long long k = c0 / N, Ja = (c0 - k * N) / Nb;
+                const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
+                for (int i = tid; i < Nb; i += 256) xs[i] = xr[i];
+                staged = key;
+            }
+        }
         for (int idx = tid; idx < MP * BN; idx += 256) Ds[idx / BN][idx % BN] = (T)0;
         __syncthreads();
         const long long c = c0 + col;
         if (c < ncols) {
             const long long k = c / N, rem = c - k * N;
             const long long Ja = rem / Nb, Ib = rem - Ja * Nb;
-            const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
             const uint32_t *__restrict__ lst = g.lb + Ib * g.Lb;
-            for (int e = part; e < g.Lb; e += PARTS) {
-                const uint32_t wd = lst[e];
-                const T v = xr[wd & 0xFFFFu];
-                Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
+            if (one_row) {
+                for (int e = part; e < g.Lb; e += PARTS) {
+                    const uint32_t wd = lst[e];
+                    const T v = xs[wd & 0xFFFFu];
+                    Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
+                }
+            } else {
+                const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
+                for (int e = part; e < g.Lb; e += PARTS) {
+                    const uint32_t wd = lst[e];
+                    const T v = xr[wd & 0xFFFFu];
+                    Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
+                }
             }
         }
         __syncthreads();
@@ -347,7 +392,19 @@ __global__ __launch_bounds__(256) void k_gga(const GfHeader *__restrict__ H,
         const uint32_t *__restrict__ lst = g.la + Ia * g.La;
         const T *__restrict__ Ek = (const T *)t.E + k * N + Ib;
         T acc = (T)0;
-        for (int e = 0; e < g.La; e++) {
+        int e = 0;
+        // eight independent loads in flight per lane
+        for (; e + 8 <= g.La; e += 8) {
+            T v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const uint32_t wd = lst[e + q];
+                v[q] = Ek[(long long)((wd >> 16) & 0xFFu) * ncols + (long long)(wd & 0xFFFFu) * Nb];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc += (lst[e + q] >> 31) ? -v[q] : v[q];
+        }
+        for (; e < g.La; e++) {
             const uint32_t wd = lst[e];
             const T v = Ek[(long long)((wd >> 16) & 0xFFu) * ncols + (long long)(wd & 0xFFFFu) * Nb];
             acc += (wd >> 31) ? -v : v;
