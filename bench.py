@@ -162,11 +162,18 @@ def roofline_dense(J, ctx, spec, seed):
     m1 = n * (n + 1) // 2
     flops = 2.0 * m1 * m1 * N + 2.0 * (na * na * nb + na * nb * nb)
     out = {}
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "sigma_dense_5a5b14_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("ndets") == N:
+            traffic = {64: t.get("bytes_per_launch_fp64"), 32: t.get("bytes_per_launch_fp32")}
     for prec, meth in ((64, J.SIGMA_GEMM64), (32, J.SIGMA_GEMM32)):
         ms = h.bench_sigma(meth, 1, 20)
         ach = flops / (ms * 1e-3) / 1e12
         out[prec] = {"bound": "mfma", "achieved": ach, "peak": PEAK_MFMA[prec],
-                     "unit": "TFLOP/s", "frac": ach / PEAK_MFMA[prec], "traffic": None,
+                     "unit": "TFLOP/s", "frac": ach / PEAK_MFMA[prec],
+                     "traffic": traffic.get(prec),
                      "kernel": "dense-string sigma %s (k_gE + k_gss + k_gga), N=%d (%da%db/%d)"
                                % ("FP64" if prec == 64 else "FP32", N, spec["na"], spec["nb"], n),
                      "ms_per_launch": ms, "algorithmic_flops": flops}

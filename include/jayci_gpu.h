@@ -214,6 +214,10 @@ typedef struct {
     int verbose;
     int nbuffer;          /* extra Ritz pairs expanded every iteration but not required to
                              converge (guards against converging onto a higher root) */
+    double etol;          /* energy criterion (0 = off): a requested root also needs
+                             |r|^2 < etol * gap, gap = distance of its Ritz value to the
+                             nearest Ritz value outside the requested set (|r|^2 / gap bounds
+                             the eigenvalue error) */
 } jci_solve_params;
 
 typedef struct {

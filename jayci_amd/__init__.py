@@ -37,7 +37,7 @@ class SolveParams(ctypes.Structure):
     _fields_ = [("maxiter", _i), ("krymin", _i), ("krymax", _i), ("nroots", _i), ("refdim", _i),
                 ("restol", _d), ("precision", _i), ("mixed_switch", _d), ("sigma_fp64", _i),
                 ("sigma_fp32", _i), ("guess", _i), ("totfrze", _d), ("verbose", _i),
-                ("nbuffer", _i)]
+                ("nbuffer", _i), ("etol", _d)]
 
 
 class SolveStats(ctypes.Structure):
@@ -351,15 +351,15 @@ class Hamiltonian:
 
     def params(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
                precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
-               sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0):
+               sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0, etol=0.0):
         return make_params(self.space.ndets, nroots, restol, maxiter, krymin, krymax, refdim,
                            precision, mixed_switch, guess, totfrze, sigma_fp64, sigma_fp32, verbose,
-                           nbuffer)
+                           nbuffer, etol)
 
     def solve(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
               precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, guess_vectors=None,
               totfrze=0.0, sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, vectors=True,
-              verbose=False, check=True, nbuffer=0):
+              verbose=False, check=True, nbuffer=0, etol=0.0):
         p = SolveParams()
         _L.jci_solve_defaults(ctypes.byref(p))
         p.nroots = nroots
@@ -376,6 +376,7 @@ class Hamiltonian:
         p.sigma_fp32 = sigma_fp32
         p.verbose = 1 if verbose else 0
         p.nbuffer = nbuffer
+        p.etol = etol
         N = self.space.ndets
         ev = np.zeros(nroots)
         vec = np.zeros((nroots, N)) if vectors else None
@@ -395,7 +396,8 @@ class Hamiltonian:
 
 def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
                 precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
-                sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0):
+                sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0,
+                etol=0.0):
     p = SolveParams()
     _L.jci_solve_defaults(ctypes.byref(p))
     p.nroots = nroots
@@ -412,6 +414,7 @@ def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=N
     p.sigma_fp32 = sigma_fp32
     p.verbose = 1 if verbose else 0
     p.nbuffer = nbuffer
+    p.etol = etol
     return p
 
 

@@ -116,10 +116,29 @@ struct jci_batch {
 };
 
 static int nblk_rows(long long N) { return (int)((N + 255) / 256); }
+// reduction work-groups per instance: 4096 rows each (enough independent loads in flight to
+// stream the vectors), at most 1024
 static int nblk_red(long long N)
 {
-    long long b = (N + 8191) / 8192;
-    return (int)std::max<long long>(1, std::min<long long>(b, 64));
+    long long b = (N + 4095) / 4096;
+    return (int)std::max<long long>(1, std::min<long long>(b, 1024));
+}
+
+// grow the partial-sum buffer (reductions) to hold `need` doubles
+static int ensure_partial(jci_batch *b, size_t need)
+{
+    if (need <= b->partial_cap) return 0;
+    JCI_CHECK(hipStreamSynchronize(b->c->stream));
+    hipFree(b->dpartial);
+    b->dpartial = nullptr;
+    size_t cap = std::max(need, b->partial_cap * 2);
+    if (hipMalloc(&b->dpartial, sizeof(double) * cap) != hipSuccess) {
+        b->partial_cap = 0;
+        jci_set_error("batch: out of device memory for %zu reduction partials", cap);
+        return -3;
+    }
+    b->partial_cap = cap;
+    return 0;
 }
 
 JCI_API int jci_batch_create(jci_ctx *c, int capacity, int max_norbs, jci_batch **out)
@@ -158,7 +177,7 @@ JCI_API int jci_batch_create(jci_ctx *c, int capacity, int max_norbs, jci_batch 
     b->out_cap = (size_t)capacity * JCI_MAXBASIS * JCI_MAXBASIS;
     JCI_CHECK(hipMalloc(&b->dout, sizeof(double) * b->out_cap));
     JCI_CHECK(hipHostMalloc(&b->hout, sizeof(double) * b->out_cap, 0));
-    b->partial_cap = (size_t)capacity * 64 * JCI_MAXBASIS * JCI_MAXBASIS;
+    b->partial_cap = (size_t)1 << 22;
     JCI_CHECK(hipMalloc(&b->dpartial, sizeof(double) * b->partial_cap));
     JCI_CHECK(hipMalloc(&b->dGH, sizeof(GfHeader)));
     JCI_CHECK(hipHostMalloc(&b->hGH, sizeof(GfHeader), 0));
@@ -358,7 +377,8 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
         pofs += (long long)d.nblk * m[t] * w[t];
         o += (size_t)m[t] * w[t];
     }
-    if ((size_t)pofs > b->partial_cap || o > b->out_cap) { jci_set_error("batch: scratch too small"); return -1; }
+    if (o > b->out_cap) { jci_set_error("batch: scratch too small"); return -1; }
+    RCHECK(ensure_partial(b, (size_t)pofs));
     RCHECK(upload_desc(b, n, blk));
     RCHECK(stage_done(b));
     RCHECK(jb_gram(b->dH, b->dD, n, b->dpartial, b->c->stream));
@@ -712,6 +732,7 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
         RCHECK(upload_coef(b, co));
         RCHECK(upload_desc(b, (int)act.size(), blk));
         RCHECK(stage_done(b));
+        RCHECK(ensure_partial(b, (size_t)pofs));
         RCHECK(jb_residual(b->dH, b->dD, (int)act.size(), b->dpartial, c->stream));
         RCHECK(download_out(b, o));
     }
@@ -736,13 +757,19 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             continue;
         }
         bool req_done = true;
-        for (int r = 0; r < s.nroots; r++)
-            if (!(s.rn[r] < s.p.restol)) req_done = false;
+        s.todo.clear();
+        for (int r = 0; r < nt; r++) {
+            bool ok = rall[r] < s.p.restol;
+            if (ok && r < s.nroots && s.p.etol > 0.0) {
+                double gap = 1e300;
+                for (int j = s.nroots; j < s.m; j++) gap = std::min(gap, fabs(s.theta[j] - s.theta[r]));
+                ok = rall[r] * rall[r] < s.p.etol * gap;
+            }
+            if (!ok && r < s.nroots) req_done = false;
+            if (!ok) s.todo.push_back(r);
+        }
         if (req_done) { s.done = 1; s.t_done = tnow(); continue; }
         if (s.iters >= s.p.maxiter) { s.done = 1; s.failed = 1; s.t_done = tnow(); continue; }
-        s.todo.clear();
-        for (int r = 0; r < nt; r++)
-            if (!(rall[r] < s.p.restol)) s.todo.push_back(r);
         // a full subspace collapses onto kc Ritz vectors; buffer corrections that do not fit
         // next to them are dropped
         s.kc = 0;

@@ -1203,14 +1203,18 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             continue;
         }
         bool req_done = true;
-        for (int r = 0; r < nroots; r++)
-            if (!(rn[r] < p->restol)) req_done = false;
-        if (req_done) { converged = 1; break; }
-        // corrections: every unconverged tracked pair (buffer pairs only while a requested
-        // root is still open, which is always the case here)
         std::vector<int> todo;
-        for (int r = 0; r < nt; r++)
-            if (!(rall[r] < p->restol)) todo.push_back(r);
+        for (int r = 0; r < nt; r++) {
+            bool ok = rall[r] < p->restol;
+            if (ok && r < nroots && p->etol > 0.0) {
+                double gap = 1e300;
+                for (int j = nroots; j < m; j++) gap = std::min(gap, fabs(theta[j] - theta[r]));
+                ok = rall[r] * rall[r] < p->etol * gap;
+            }
+            if (!ok && r < nroots) req_done = false;
+            if (!ok) todo.push_back(r);
+        }
+        if (req_done) { converged = 1; break; }
         if (it == p->maxiter) break;
         int k = (int)todo.size();
         // compact the unconverged corrections to the front of T

@@ -49,7 +49,7 @@ void jci_binom_table(unsigned long long *tab /* 65*65 */);
 #define JCI_API extern "C" __attribute__((visibility("default")))
 
 // ---- device context (jci.cpp) ------------------------------------------------------------
-#define JCI_PARTIAL_BLOCKS 512
+#define JCI_PARTIAL_BLOCKS 2048
 struct jci_ctx {
     int device;
     int refs;                   // spaces holding this context

@@ -648,9 +648,11 @@ __global__ void k_gram_reduce(int m, int w, int nblocks, const double *__restric
     if (lane == 0) out[i * w + j] = s;
 }
 
+// reduction work-groups: 2048 rows each (the chunks are walked sequentially, so shorter
+// chunks put more independent loads in flight), at most `cap`
 static int reduction_blocks(long long N, int cap)
 {
-    long long b = (N + 8191) / 8192;
+    long long b = (N + 2047) / 2048;
     if (b > cap) b = cap;
     return (int)(b < 1 ? 1 : b);
 }

@@ -45,13 +45,16 @@ def ndets_of(spec):
     return None
 
 
-def solve_params(spec, N, restol=1e-6, nbuffer=1):
+def solve_params(spec, N, restol=1e-6, nbuffer=1, etol=1e-9, kfac=8):
     """Davidson settings of a request (the same for the served and the one-at-a-time path):
-    block of nroots + nbuffer tracked pairs, subspace up to 4x that (at least 12)."""
+    block of nroots + nbuffer tracked pairs, subspace up to kfac times that (at least 12);
+    residual below restol and |r|^2 / gap below etol (eigenvalue error well under 1e-8 Ha
+    also next to a close unrequested state)."""
     nr = spec["nroots"]
     nt = nr + nbuffer
-    return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N), krymax=min(max(4 * nt, 12), N),
-                refdim=min(64, N), guess=GUESS_DIAG, totfrze=SYNTH_ENUC, nbuffer=nbuffer)
+    return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N),
+                krymax=min(max(kfac * nt, 12), N, 64), refdim=min(64, N), guess=GUESS_DIAG,
+                totfrze=SYNTH_ENUC, nbuffer=nbuffer, etol=etol)
 
 
 class Request:
