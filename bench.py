@@ -67,6 +67,24 @@ def make_instance(J, ctx, spec, seed, flags):
     return sp, J.Hamiltonian(sp, m1, m2, flags)
 
 
+class c_stdout_to_stderr:
+    """Send what C code writes to file descriptor 1 to stderr while active (the reference
+    prints its CI-expansion banner with printf; stdout carries only the JSON line)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
 def fci_pairs(spec):
     """(N, unique off-diagonal pairs) of a full-CI space: c = 2a(n-a) + 2C(a,2)C(n-a,2)
     + (a(n-a))^2 connections per row for a = alpha = beta electrons."""
@@ -92,8 +110,9 @@ def cpu_baseline(J, sample, seed, work):
         return None
     os.environ.setdefault("OMP_SCHEDULE", "dynamic")
     m1, m2 = J.synth_integrals(sample["norbs"], seed)
-    rs = ref.RefSpace(sample["na"], sample["nb"], sample["norbs"], sample["ndocc"],
-                      sample["nactv"], sample["xlvl"])
+    with c_stdout_to_stderr():
+        rs = ref.RefSpace(sample["na"], sample["nb"], sample["norbs"], sample["ndocc"],
+                          sample["nactv"], sample["xlvl"])
     hmap = ref._libc.malloc(ctypes.sizeof(ref.RowMap) * rs.ndets)
     gen = J.lib().generate_cimap
     gen.restype = ctypes.c_int
@@ -101,9 +120,10 @@ def cpu_baseline(J, sample, seed, work):
     if gen(ctypes.cast(rs.dlist, ctypes.c_void_p), rs.ndets, rs.ninto, hmap) != 0:
         return None
     v = np.random.default_rng(0).standard_normal(rs.ndets)
-    t0 = time.time()
-    rs.compute_hv(m1, m2, v, hmap)
-    t = time.time() - t0
+    with c_stdout_to_stderr():
+        t0 = time.time()
+        rs.compute_hv(m1, m2, v, hmap)
+        t = time.time() - t0
     dealloc = J.lib().deallocate_cimap
     dealloc.argtypes = [ctypes.c_void_p, ctypes.c_int]
     dealloc(hmap, rs.ndets)

@@ -346,6 +346,7 @@ static int build_gf(jci_ham *h)
     const int sa = s->na * (n - s->na), sb = s->nb * (n - s->nb);
     if (h->nexc1[0] != na * sa || h->nexc1[1] != nb * sb) return 0;
     const int La = sa + s->na, Lb = sb + s->nb;
+    if (La > GF_MAXLIST || Lb > GF_MAXLIST) return 0;
     JCI_CHECK(hipMalloc(&h->d_gfH64[0], sizeof(double) * na * na));
     JCI_CHECK(hipMalloc(&h->d_gfH32[0], sizeof(float) * na * na));
     RCHECK(jk_gf_dense(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], h->d_v1[0], h->d_off2[0],

@@ -24,6 +24,7 @@
 
 #define GF_MAXMP 128          // padded orbital-pair count (n <= 15)
 #define GF_MAXSTR 65535       // strings per spin addressable by a 16-bit list entry
+#define GF_MAXLIST 64         // replacement-list entries per string: b(n-b+1) <= 64 for n <= 15
 
 // list entry: bits 0..15 target string, 16..23 orbital pair, bit 31 sign
 struct GemmFci {

@@ -274,14 +274,37 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
     __shared__ T Ds[GF_MAXMP][BN + 1];
     __shared__ T xs[GF_XROW];
     constexpr int NCT = BN / 16, PARTS = 256 / BN, KSMAX = GF_MAXMP / 4;
+    constexpr int EMAX = (GF_MAXLIST + PARTS - 1) / PARTS;   // list entries per thread
     const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
     const int n = H->n;
     const long long total = H->total[GF_E];
     const int col = tid % BN, part = tid / BN;
+    const long long u0 = total * blockIdx.x / gridDim.x, u1 = total * (blockIdx.x + 1) / gridDim.x;
+    if (u0 >= u1) return;
+    for (int idx = tid; idx < GF_MAXMP * (BN + 1); idx += 256) (&Ds[0][0])[idx] = (T)0;
+    // beta-list entries of this thread's column for a tile (issued one tile ahead, so their
+    // global loads overlap the products of the current tile)
+    uint32_t lcur[EMAX], lnxt[EMAX];
+    int ncur = 0, nnxt = 0;
+    auto fetch = [&](long long u, uint32_t (&dst)[EMAX], int &cnt) {
+        cnt = 0;
+        if (u >= u1) return;
+        const GfTask &t = tasks[find_task(tasks, n, u, GF_E)];
+        const long long N = t.g.na * t.g.nb, ncols = N * t.K;
+        const long long c = (u - t.base[GF_E]) * BN + col;
+        if (c >= ncols) return;
+        const long long Ib = (c % N) % t.g.nb;
+        const uint32_t *__restrict__ lst = t.g.lb + Ib * t.g.Lb;
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) {
+            const int e = part + i * PARTS;
+            if (e < t.g.Lb) { dst[i] = lst[e]; cnt = i + 1; }
+        }
+    };
+    fetch(u0, lcur, ncur);
     T a[2][KSMAX];
     int cur = -1;
     long long staged = -1;        // (task, k, Ja) of the row in xs, as task * 2^40 + k * N + Ja*Nb
-    const long long u0 = total * blockIdx.x / gridDim.x, u1 = total * (blockIdx.x + 1) / gridDim.x;
     for (long long u = u0; u < u1; u++) {
         const int ti = find_task(tasks, n, u, GF_E);
         const GfTask &t = tasks[ti];
@@ -314,29 +337,31 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
                 staged = key;
             }
         }
-        for (int idx = tid; idx < MP * BN; idx += 256) Ds[idx / BN][idx % BN] = (T)0;
-        __syncthreads();
+        __syncthreads();   // row staged; the previous tile's entries are cleared
         const long long c = c0 + col;
         if (c < ncols) {
-            const long long k = c / N, rem = c - k * N;
-            const long long Ja = rem / Nb, Ib = rem - Ja * Nb;
-            const uint32_t *__restrict__ lst = g.lb + Ib * g.Lb;
             if (one_row) {
-                for (int e = part; e < g.Lb; e += PARTS) {
-                    const uint32_t wd = lst[e];
-                    const T v = xs[wd & 0xFFFFu];
-                    Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
-                }
+#pragma unroll
+                for (int i = 0; i < EMAX; i++)
+                    if (i < ncur) {
+                        const uint32_t wd = lcur[i];
+                        const T v = xs[wd & 0xFFFFu];
+                        Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
+                    }
             } else {
+                const long long k = c / N, Ja = (c - k * N) / Nb;
                 const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
-                for (int e = part; e < g.Lb; e += PARTS) {
-                    const uint32_t wd = lst[e];
-                    const T v = xr[wd & 0xFFFFu];
-                    Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
-                }
+#pragma unroll
+                for (int i = 0; i < EMAX; i++)
+                    if (i < ncur) {
+                        const uint32_t wd = lcur[i];
+                        const T v = xr[wd & 0xFFFFu];
+                        Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
+                    }
             }
         }
-        __syncthreads();
+        __syncthreads();   // D tile complete
+        fetch(u + 1, lnxt, nnxt);
         typename Mf<T>::acc acc[2][NCT];
 #pragma unroll
         for (int i = 0; i < 2; i++)
@@ -370,7 +395,14 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
                     if (P < g.m1 && cc < ncols) E[(long long)P * ncols + cc] = acc[i][j][v];
                 }
         }
-        __syncthreads();
+        __syncthreads();   // the products have read the D tile
+        // clear exactly the entries this thread wrote (the rest of Ds stays zero)
+#pragma unroll
+        for (int i = 0; i < EMAX; i++)
+            if (i < ncur) Ds[(lcur[i] >> 16) & 0xFFu][col] = (T)0;
+#pragma unroll
+        for (int i = 0; i < EMAX; i++) lcur[i] = lnxt[i];
+        ncur = nnxt;
     }
 }
 
