@@ -23,6 +23,19 @@
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// device-memory (global address space) views of pointers read from descriptors, so their
+// loads are global loads (vector-memory counter only) rather than flat ones, which a later
+// LDS wait would also have to drain
+template <typename T> __device__ __forceinline__ const __attribute__((address_space(1))) T *
+gptr(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T> __device__ __forceinline__ __attribute__((address_space(1))) T *gptr_w(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // 16x16 output tile, 4-deep reduction per instruction.  Operand lanes: A[l % 16][l / 16],
 // B[l / 16][l % 16]; accumulator element v of lane l sits in column l % 16 and row row(l, v).
 template <typename T> struct Mf;
@@ -174,18 +187,19 @@ __device__ __forceinline__ void mma_tile(typename Mf<T>::acc (&acc)[2][2], const
     const int ar = tid >> 2, ak = (tid & 3) * 4;
     const int bk = tid >> 4, bc = (tid & 15) * 4;
     const long long gm = m0 + ar;
+    const auto Ag = gptr(A), Bg = gptr(B);
     T ra[4], rb[4];
     auto gload = [&](long long k0) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const long long gk = k0 + ak + j;
-            ra[j] = (gm < M && gk < Kr) ? A[gm * lda + gk] : (T)0;
+            ra[j] = (gm < M && gk < Kr) ? Ag[gm * lda + gk] : (T)0;
         }
         const long long gk = k0 + bk;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const long long gn = n0 + bc + j;
-            rb[j] = (gk < Kr && gn < N) ? B[gk * ldb + gn] : (T)0;
+            rb[j] = (gk < Kr && gn < N) ? Bg[gk * ldb + gn] : (T)0;
         }
     };
     auto sstore = [&](int buf) {
@@ -294,7 +308,7 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
         const long long c = (u - t.base[GF_E]) * BN + col;
         if (c >= ncols) return;
         const long long Ib = (c % N) % t.g.nb;
-        const uint32_t *__restrict__ lst = t.g.lb + Ib * t.g.Lb;
+        const auto lst = gptr(t.g.lb + Ib * t.g.Lb);
 #pragma unroll
         for (int i = 0; i < EMAX; i++) {
             const int e = part + i * PARTS;
@@ -340,25 +354,20 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
         __syncthreads();   // row staged; the previous tile's entries are cleared
         const long long c = c0 + col;
         if (c < ncols) {
+            // all reads first (independent, one wait), then the scatter into the D tile
+            T v[EMAX];
             if (one_row) {
 #pragma unroll
-                for (int i = 0; i < EMAX; i++)
-                    if (i < ncur) {
-                        const uint32_t wd = lcur[i];
-                        const T v = xs[wd & 0xFFFFu];
-                        Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
-                    }
+                for (int i = 0; i < EMAX; i++) v[i] = i < ncur ? xs[lcur[i] & 0xFFFFu] : (T)0;
             } else {
                 const long long k = c / N, Ja = (c - k * N) / Nb;
-                const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
+                const auto xr = gptr(xin<T>(t, (int)k) + Ja * Nb);
 #pragma unroll
-                for (int i = 0; i < EMAX; i++)
-                    if (i < ncur) {
-                        const uint32_t wd = lcur[i];
-                        const T v = xr[wd & 0xFFFFu];
-                        Ds[(wd >> 16) & 0xFFu][col] = (wd >> 31) ? -v : v;
-                    }
+                for (int i = 0; i < EMAX; i++) v[i] = i < ncur ? xr[lcur[i] & 0xFFFFu] : (T)0;
             }
+#pragma unroll
+            for (int i = 0; i < EMAX; i++)
+                if (i < ncur) Ds[(lcur[i] >> 16) & 0xFFu][col] = (lcur[i] >> 31) ? -v[i] : v[i];
         }
         __syncthreads();   // D tile complete
         fetch(u + 1, lnxt, nnxt);
@@ -381,7 +390,7 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
                     }
             }
         }
-        T *__restrict__ E = (T *)t.E;
+        const auto E = gptr_w((T *)t.E);
 #pragma unroll
         for (int i = 0; i < 2; i++) {
             const int rt = w + 4 * i;
@@ -421,8 +430,8 @@ __global__ __launch_bounds__(256) void k_gga(const GfHeader *__restrict__ H,
         if (idx >= ncols) continue;
         const long long k = idx / N, rem = idx - k * N;
         const long long Ia = rem / Nb, Ib = rem - Ia * Nb;
-        const uint32_t *__restrict__ lst = g.la + Ia * g.La;
-        const T *__restrict__ Ek = (const T *)t.E + k * N + Ib;
+        const auto lst = gptr(g.la + Ia * g.La);
+        const auto Ek = gptr((const T *)t.E + k * N + Ib);
         T acc = (T)0;
         int e = 0;
         // eight independent loads in flight per lane
