@@ -256,6 +256,9 @@ int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32, d
 /* times[] above has JCI_SIGMA_GEMM32 + 1 entries */
 /* beta strings per tile of the matrix-free sigma (0 without excitation tables) */
 int jci_ham_fast_width(const jci_ham *h);
+/* column-tile widths of the dense-string sigma's opposite-spin product (FP64: 16 or 32,
+ * FP32: 32 or 64; the autotuner sets them): a positive *bn sets, then both are returned */
+int jci_ham_gemm_tiles(jci_ham *h, int *bn64, int *bn32);
 
 /* dense symmetric eigensolver (cyclic Jacobi, FP64) used for the Davidson subspace:
  * eigenvalues ascending, eigenvector i in vec[i*n + 0..n-1] (largest component positive) */
@@ -275,6 +278,8 @@ int jci_batch_set_graphs(jci_batch *b, int on);
  * allocated at the first such sigma, grown to the largest single vector when smaller);
  * a fuller step runs its sigma in several rounds */
 int jci_batch_set_pool(jci_batch *b, long long bytes);
+/* the same tile widths for the batch's dense-string sigma (values <= 0 keep the current) */
+int jci_batch_set_tiles(jci_batch *b, int bn64, int bn32);
 /* device bytes a slot of ndets determinants takes (basis, sigma and staging vectors) */
 long long jci_batch_bytes(long long ndets, int krymax);
 int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, int *slot);

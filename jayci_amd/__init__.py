@@ -90,6 +90,8 @@ _sig("jci_batch_create", _i, [_P, _i, _i, ctypes.POINTER(_P)])
 _sig("jci_batch_destroy", None, [_P])
 _sig("jci_batch_set_graphs", _i, [_P, _i])
 _sig("jci_batch_set_pool", _i, [_P, _ll])
+_sig("jci_batch_set_tiles", _i, [_P, _i, _i])
+_sig("jci_ham_gemm_tiles", _i, [_P, ctypes.POINTER(_i), ctypes.POINTER(_i)])
 _sig("jci_batch_bytes", _ll, [_ll, _i])
 _sig("jci_batch_add", _i, [_P, _P, ctypes.POINTER(SolveParams), ctypes.POINTER(_i)])
 _sig("jci_batch_step", _i, [_P, ctypes.POINTER(_i)])
@@ -335,6 +337,13 @@ class Hamiltonian:
         return b64.value, b32.value, {m: t[m] for m in range(NMETHODS) if t[m] > 0}, \
             _L.jci_ham_fast_width(self._h)
 
+    def gemm_tiles(self, bn64=0, bn32=0):
+        """E tile widths of the dense-string sigma; positive arguments set them."""
+        a, b = _i(bn64), _i(bn32)
+        _check(_L.jci_ham_gemm_tiles(self._h, ctypes.byref(a), ctypes.byref(b)),
+               "jci_ham_gemm_tiles")
+        return a.value, b.value
+
     def sigma(self, x, method=SIGMA_AUTO):
         """H x for x of shape (N,) or (k, N) (vector-major); returns the same shape."""
         x = np.asarray(x, dtype=np.float64)
@@ -431,6 +440,9 @@ class Batch:
             _L.jci_batch_set_graphs(h, 0)
         if pool_bytes is not None:
             _check(_L.jci_batch_set_pool(h, int(pool_bytes)), "jci_batch_set_pool")
+
+    def set_tiles(self, bn64=0, bn32=0):
+        _check(_L.jci_batch_set_tiles(self._h, bn64, bn32), "jci_batch_set_tiles")
 
     def close(self):
         if getattr(self, "_h", None):

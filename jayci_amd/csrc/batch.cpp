@@ -107,6 +107,7 @@ struct jci_batch {
     GfTask *dGT, *hGT;
     void *epool;
     size_t epool_bytes, epool_want;
+    int bn64, bn32;    // E tile widths of the dense-string sigma
     long long steps;
     double sigma_seconds;
     // host wall time per step phase: 0 sigma (all of run_sigma), 1 Gram + Rayleigh-Ritz,
@@ -186,6 +187,8 @@ JCI_API int jci_batch_create(jci_ctx *c, int capacity, int max_norbs, jci_batch 
     b->epool = nullptr;
     b->epool_bytes = 0;
     b->epool_want = (size_t)24 << 30;
+    b->bn64 = GF_BN64_DEFAULT;
+    b->bn32 = GF_BN32_DEFAULT;
     *out = b;
     return 0;
 }
@@ -221,6 +224,20 @@ JCI_API void jci_batch_destroy(jci_batch *b)
 }
 
 JCI_API int jci_batch_set_graphs(jci_batch *b, int on) { b->use_graphs = on; return 0; }
+
+JCI_API int jci_batch_set_tiles(jci_batch *b, int bn64, int bn32)
+{
+    if ((bn64 > 0 && !jk_gf_bn_valid(64, bn64)) || (bn32 > 0 && !jk_gf_bn_valid(32, bn32))) {
+        jci_set_error("batch: tile widths %d / %d not available", bn64, bn32);
+        return -1;
+    }
+    hipStreamSynchronize(b->c->stream);
+    if (bn64 > 0) b->bn64 = bn64;
+    if (bn32 > 0) b->bn32 = bn32;
+    for (auto &g : b->graphs) hipGraphExecDestroy(g.exec);   // recorded with the old widths
+    b->graphs.clear();
+    return 0;
+}
 
 JCI_API int jci_batch_set_pool(jci_batch *b, long long bytes)
 {
@@ -461,7 +478,7 @@ static int run_sigma_gf(jci_batch *b, const std::vector<int> &ids, int prec)
             t.ldy = s.ld;
             t.E = (char *)b->epool + used;
             long long u[4];
-            jk_gf_units(prec, t, u);
+            jk_gf_units(prec, prec == 64 ? b->bn64 : b->bn32, t, u);
             for (int q = 0; q < 4; q++) { t.base[q] = base[q]; base[q] += u[q]; }
             used += (per * k + 255) / 256 * 256;
             who.push_back(i);
@@ -479,7 +496,8 @@ static int run_sigma_gf(jci_batch *b, const std::vector<int> &ids, int prec)
             if (!ex) {
                 hipGraph_t g;
                 JCI_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                int rc = jk_gf_group(prec, b->dGH, b->dGT, nullptr, c->stream);
+                int rc = jk_gf_group(prec, prec == 64 ? b->bn64 : b->bn32, b->dGH, b->dGT, nullptr,
+                                     c->stream);
                 hipError_t e = hipStreamEndCapture(c->stream, &g);
                 if (rc) return rc;
                 if (e != hipSuccess) { jci_set_error("batch: graph capture failed: %s", hipGetErrorString(e)); return -2; }
@@ -489,7 +507,8 @@ static int run_sigma_gf(jci_batch *b, const std::vector<int> &ids, int prec)
             }
             JCI_CHECK(hipGraphLaunch(ex, c->stream));
         } else {
-            RCHECK(jk_gf_group(prec, b->dGH, b->dGT, b->hGH, c->stream));
+            RCHECK(jk_gf_group(prec, prec == 64 ? b->bn64 : b->bn32, b->dGH, b->dGT, b->hGH,
+                               c->stream));
         }
         JCI_CHECK(hipStreamSynchronize(c->stream));
         b->sigma_seconds += tnow() - t0;

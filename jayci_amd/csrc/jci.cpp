@@ -283,6 +283,7 @@ struct jci_ham {
     GfTask *h_gft, *d_gft;
     GfHeader *h_gfh, *d_gfh;
     int gf_valid[JCI_MAXK];
+    int gf_bn64, gf_bn32;       // E tile widths (autotuned)
 };
 
 static size_t m1_len(int n) { return (size_t)n * (n + 1) / 2; }
@@ -385,6 +386,8 @@ static int build_gf(jci_ham *h)
     g.M32 = h->d_gfM32;
     g.la = h->d_gfla;
     g.lb = h->d_gflb;
+    h->gf_bn64 = GF_BN64_DEFAULT;
+    h->gf_bn32 = GF_BN32_DEFAULT;
     h->have_gf = 1;
     return 0;
 }
@@ -670,7 +673,8 @@ static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, d
         t.ldy = ldy;
         t.E = h->d_gfE;
         hh.n = 1;
-        jk_gf_units(prec, t, hh.total);
+        const int bn = prec == 64 ? h->gf_bn64 : h->gf_bn32;
+        jk_gf_units(prec, bn, t, hh.total);
         if (!h->gf_valid[ci] || memcmp(&t, &h->h_gft[ci], sizeof(t)) ||
             memcmp(&hh, &h->h_gfh[ci], sizeof(hh))) {
             JCI_CHECK(hipStreamSynchronize(c->stream));   // earlier uploads of the slot done
@@ -680,7 +684,7 @@ static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, d
             JCI_CHECK(hipMemcpyAsync(h->d_gfh + ci, h->h_gfh + ci, sizeof(hh), hipMemcpyHostToDevice, c->stream));
             h->gf_valid[ci] = 1;
         }
-        RCHECK(jk_gf_group(prec, h->d_gfh + ci, h->d_gft + ci, &hh, c->stream));
+        RCHECK(jk_gf_group(prec, bn, h->d_gfh + ci, h->d_gft + ci, &hh, c->stream));
     }
     return 0;
 }
@@ -866,8 +870,23 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_FCI32, nvec, reps, &t[JCI_SIGMA_FCI32]));
     }
     if (h->have_gf) {
-        RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM64, nvec, reps, &t[JCI_SIGMA_GEMM64]));
-        RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM32, nvec, reps, &t[JCI_SIGMA_GEMM32]));
+        // E tile widths: keep the fastest per precision
+        const int w64[2] = {16, 32}, w32[2] = {32, 64};
+        double b64 = 1e300, b32 = 1e300;
+        int k64 = h->gf_bn64, k32 = h->gf_bn32;
+        for (int i = 0; i < 2; i++) {
+            double ms = 0;
+            h->gf_bn64 = w64[i];
+            RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM64, nvec, reps, &ms));
+            if (ms < b64) { b64 = ms; k64 = w64[i]; }
+            h->gf_bn32 = w32[i];
+            RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM32, nvec, reps, &ms));
+            if (ms < b32) { b32 = ms; k32 = w32[i]; }
+        }
+        h->gf_bn64 = k64;
+        h->gf_bn32 = k32;
+        t[JCI_SIGMA_GEMM64] = b64;
+        t[JCI_SIGMA_GEMM32] = b32;
     }
     if (h->d_val64 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS64, nvec, reps, &t[JCI_SIGMA_PAIRS64]));
     if (h->d_val32 && !h->upper) RCHECK(jci_bench_sigma(h, JCI_SIGMA_PAIRS32, nvec, reps, &t[JCI_SIGMA_PAIRS32]));
@@ -887,6 +906,22 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
     if (best64) *best64 = b64;
     if (best32) *best32 = b32;
     if (times) for (int m = 0; m <= JCI_SIGMA_GEMM32; m++) times[m] = t[m];
+    return 0;
+}
+
+JCI_API int jci_ham_gemm_tiles(jci_ham *h, int *bn64, int *bn32)
+{
+    if (!h->have_gf) { jci_set_error("ham: no dense-string sigma"); return -1; }
+    if (bn64 && *bn64 > 0) {
+        if (!jk_gf_bn_valid(64, *bn64)) { jci_set_error("ham: FP64 tile width %d not available", *bn64); return -1; }
+        h->gf_bn64 = *bn64;
+    }
+    if (bn32 && *bn32 > 0) {
+        if (!jk_gf_bn_valid(32, *bn32)) { jci_set_error("ham: FP32 tile width %d not available", *bn32); return -1; }
+        h->gf_bn32 = *bn32;
+    }
+    if (bn64) *bn64 = h->gf_bn64;
+    if (bn32) *bn32 = h->gf_bn32;
     return 0;
 }
 

@@ -62,12 +62,16 @@ struct GfHeader {
 #define GF_E 2
 #define GF_GA 3
 
-// work units of one task per kernel for the given precision
-void jk_gf_units(int prec, const GfTask &t, long long units[4]);
+// column-tile width of the E kernel (autotuned): FP64 16 or 32, FP32 32 or 64
+#define GF_BN64_DEFAULT 32
+#define GF_BN32_DEFAULT 64
+int jk_gf_bn_valid(int prec, int bn);
+// work units of one task per kernel for the given precision and E tile width
+void jk_gf_units(int prec, int bn, const GfTask &t, long long units[4]);
 // the grouped sigma: (FP32: conversion) -> E -> same-spin product -> gather.  hdr_host
 // (may be NULL) sizes the grids to the work; NULL launches the fixed persistent grids
 // that read the totals from d_hdr (graph capture).
-int jk_gf_group(int prec, const GfHeader *d_hdr, const GfTask *d_tasks,
+int jk_gf_group(int prec, int bn, const GfHeader *d_hdr, const GfTask *d_tasks,
                 const GfHeader *hdr_host, hipStream_t st);
 
 // build the dense same-spin matrix of one spin from the excitation tables

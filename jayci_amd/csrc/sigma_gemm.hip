@@ -486,13 +486,14 @@ int jk_gf_m2pad(int m1, int MP, const double *m2sq, double *o64, float *o32, hip
     return 0;
 }
 
-#define GF_BN64 32
-#define GF_BN32 64
+int jk_gf_bn_valid(int prec, int bn)
+{
+    return prec == 64 ? (bn == 16 || bn == 32) : (bn == 32 || bn == 64);
+}
 
-void jk_gf_units(int prec, const GfTask &t, long long units[4])
+void jk_gf_units(int prec, int bn, const GfTask &t, long long units[4])
 {
     const long long na = t.g.na, nb = t.g.nb, N = na * nb, cols = N * t.K;
-    const int bn = prec == 64 ? GF_BN64 : GF_BN32;
     units[GF_CV] = prec == 64 ? 0 : (cols + 255) / 256;
     units[GF_SS] = (long long)t.K * ((na + 63) / 64) * ((nb + 63) / 64);
     units[GF_E] = (cols + bn - 1) / bn;
@@ -522,9 +523,16 @@ static int gf_group_t(const GfHeader *d_hdr, const GfTask *d_tasks, const GfHead
     return 0;
 }
 
-int jk_gf_group(int prec, const GfHeader *d_hdr, const GfTask *d_tasks, const GfHeader *hdr_host,
-                hipStream_t st)
+int jk_gf_group(int prec, int bn, const GfHeader *d_hdr, const GfTask *d_tasks,
+                const GfHeader *hdr_host, hipStream_t st)
 {
-    if (prec == 64) return gf_group_t<double, GF_BN64>(d_hdr, d_tasks, hdr_host, st);
-    return gf_group_t<float, GF_BN32>(d_hdr, d_tasks, hdr_host, st);
+    if (prec == 64) {
+        if (bn == 16) return gf_group_t<double, 16>(d_hdr, d_tasks, hdr_host, st);
+        if (bn == 32) return gf_group_t<double, 32>(d_hdr, d_tasks, hdr_host, st);
+    } else {
+        if (bn == 32) return gf_group_t<float, 32>(d_hdr, d_tasks, hdr_host, st);
+        if (bn == 64) return gf_group_t<float, 64>(d_hdr, d_tasks, hdr_host, st);
+    }
+    jci_set_error("gf_sigma: E tile width %d not available for FP%d", bn, prec);
+    return -1;
 }
