@@ -656,7 +656,8 @@ static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, d
     const int Kc = (int)std::max(1.0, std::min((double)K, 8e9 / ((double)h->gf.m1 * N * esz)));
     for (int b = 0, ci = 0; b < K; b += Kc, ci++) {
         const int k = std::min(Kc, K - b);
-        RCHECK(grow(&h->d_gfE, &h->gfE_bytes, (size_t)h->gf.m1 * k * N * esz, "the intermediate"));
+        RCHECK(grow(&h->d_gfE, &h->gfE_bytes, (size_t)h->gf.m1 * jk_gf_lde((long long)k * N) * esz,
+                    "the intermediate"));
         if (prec == 32)
             RCHECK(grow((void **)&h->d_gfX32, &h->gfX32_len, sizeof(float) * (size_t)k * N,
                         "FP32 staging"));
@@ -672,6 +673,7 @@ static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, d
         t.y = y + (size_t)b * ldy;
         t.ldy = ldy;
         t.E = h->d_gfE;
+        t.lde = jk_gf_lde((long long)k * N);
         hh.n = 1;
         const int bn = prec == 64 ? h->gf_bn64 : h->gf_bn32;
         jk_gf_units(prec, bn, t, hh.total);

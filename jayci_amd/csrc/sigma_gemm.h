@@ -50,9 +50,14 @@ struct GfTask {
     float *x32;                  // FP32 copies [K][N] (FP32 launches)
     double *y;                   // output columns, vector k at y + k*ldy (overwritten)
     long long ldy;
-    void *E;                     // intermediate, m1 rows of K*N elements
+    void *E;                     // intermediate, m1 rows of K*N elements, row stride lde
+    long long lde;               // K*N rounded up to GF_EALIGN elements (jk_gf_lde)
     long long base[4];           // first unit: conversion, same-spin, E, gather
 };
+// E rows start on 512-byte boundaries (FP64) / 256-byte (FP32): the tile stores of the E
+// kernel then write whole 64-byte sectors
+#define GF_EALIGN 64
+static inline long long jk_gf_lde(long long ncols) { return (ncols + GF_EALIGN - 1) / GF_EALIGN * GF_EALIGN; }
 struct GfHeader {
     int n;                       // tasks
     long long total[4];          // units per kernel
@@ -64,7 +69,7 @@ struct GfHeader {
 
 // column-tile width of the E kernel (autotuned): FP64 16 or 32, FP32 32 or 64
 #define GF_BN64_DEFAULT 32
-#define GF_BN32_DEFAULT 64
+#define GF_BN32_DEFAULT 32
 int jk_gf_bn_valid(int prec, int bn);
 // work units of one task per kernel for the given precision and E tile width
 void jk_gf_units(int prec, int bn, const GfTask &t, long long units[4]);

@@ -274,57 +274,139 @@ __global__ __launch_bounds__(256) void k_gss(const GfHeader *__restrict__ H,
     }
 }
 
+// The D tile sits in LDS column-major, Dt[column][Q], and the reduction index of the
+// products is permuted: step ks, lane group kk = l / 16 takes Q = kk * KS + ks (the (P|Q)
+// fragments use the same order).  A lane's B values for VK consecutive steps are then
+// adjacent and come in one 16-byte LDS read.  Row stride GF_LDT elements (a multiple of
+// 16 bytes, 4 banks past a multiple of 64 banks).
+template <typename T> struct GeL;
+template <> struct GeL<double> {
+    static constexpr int LDT = GF_MAXMP + 2;
+    static constexpr int VK = 2;
+    typedef double2 vec;
+    static __device__ __forceinline__ double at(const vec &v, int q) { return q ? v.y : v.x; }
+};
+template <> struct GeL<float> {
+    static constexpr int LDT = GF_MAXMP + 4;
+    static constexpr int VK = 4;
+    typedef float4 vec;
+    static __device__ __forceinline__ float at(const vec &v, int q)
+    {
+        return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+    }
+};
+
+// Products of one D tile: acc[i][j] += A(row tile w + 4i) . D(column tile j) over KS
+// reduction steps, straight-line code, the B vectors of the next group of steps read while
+// the products of the current group run.
+template <typename T, int BN, int KS, int NRT>
+__device__ __forceinline__ void gE_mma(typename Mf<T>::acc (&acc)[2][BN / 16],
+                                       const T (&a)[2][GF_MAXMP / 4],
+                                       const T (*Dt)[GeL<T>::LDT], int l)
+{
+    constexpr int NCT = BN / 16, VK = GeL<T>::VK, NG = KS / VK;
+    typedef typename GeL<T>::vec V;
+    const int kk = l / 16;
+    V b[2][NCT];
+#pragma unroll
+    for (int j = 0; j < NCT; j++) b[0][j] = *(const V *)&Dt[j * 16 + l % 16][kk * KS];
+#pragma unroll
+    for (int gi = 0; gi < NG; gi++) {
+        const int cb = gi & 1;
+        if (gi + 1 < NG) {
+#pragma unroll
+            for (int j = 0; j < NCT; j++)
+                b[cb ^ 1][j] = *(const V *)&Dt[j * 16 + l % 16][kk * KS + (gi + 1) * VK];
+        }
+#pragma unroll
+        for (int q = 0; q < VK; q++)
+#pragma unroll
+            for (int i = 0; i < NRT; i++)
+#pragma unroll
+                for (int j = 0; j < NCT; j++)
+                    acc[i][j] = Mf<T>::mma(a[i][gi * VK + q], GeL<T>::at(b[cb][j], q), acc[i][j]);
+    }
+}
+
+// the reduction depth KS = MP / 4 is a multiple of 4 up to GF_MAXMP / 4
+template <typename T, int BN, int NRT>
+__device__ __forceinline__ void gE_mma_ks(int KS, typename Mf<T>::acc (&acc)[2][BN / 16],
+                                          const T (&a)[2][GF_MAXMP / 4],
+                                          const T (*Dt)[GeL<T>::LDT], int l)
+{
+    static_assert(GF_MAXMP == 128, "reduction depths below cover MP <= 128");
+    switch (KS) {
+    case 4: gE_mma<T, BN, 4, NRT>(acc, a, Dt, l); break;
+    case 8: gE_mma<T, BN, 8, NRT>(acc, a, Dt, l); break;
+    case 12: gE_mma<T, BN, 12, NRT>(acc, a, Dt, l); break;
+    case 16: gE_mma<T, BN, 16, NRT>(acc, a, Dt, l); break;
+    case 20: gE_mma<T, BN, 20, NRT>(acc, a, Dt, l); break;
+    case 24: gE_mma<T, BN, 24, NRT>(acc, a, Dt, l); break;
+    case 28: gE_mma<T, BN, 28, NRT>(acc, a, Dt, l); break;
+    default: gE_mma<T, BN, 32, NRT>(acc, a, Dt, l); break;
+    }
+}
+
 // E[P][c] = sum_Q (P|Q) D[Q][c] over columns c = (k, Ja, Ib): the tile of D is built in LDS
 // from the beta lists (each (Q, column) receives at most one entry) and multiplied by the
 // (P|Q) fragments that every wave keeps in registers (wave w: row tiles w and w + 4),
 // reloaded when the task changes.  Each work-group takes a contiguous run of column tiles,
 // so consecutive tiles share the vector row X_k[Ja]: it is staged in LDS once (rows of up to
-// GF_XROW strings) and the beta-list gathers read it there.
+// GF_XROW strings) and the beta-list gathers read it there.  Columns past the end of the
+// last tile repeat the last column (their products are not stored), so every lane of a part
+// handles the same number of list entries.
 #define GF_XROW 4096
 template <typename T, int BN>
-__global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
-                                            const GfTask *__restrict__ tasks)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gE(
+    const GfHeader *__restrict__ H, const GfTask *__restrict__ tasks)
 {
-    __shared__ T Ds[GF_MAXMP][BN + 1];
+    constexpr int LDT = GeL<T>::LDT;
+    __shared__ __align__(16) T Dt[BN][LDT];
     __shared__ T xs[GF_XROW];
     constexpr int NCT = BN / 16, PARTS = 256 / BN, KSMAX = GF_MAXMP / 4;
     constexpr int EMAX = (GF_MAXLIST + PARTS - 1) / PARTS;   // list entries per thread
-    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int n = H->n;
     const long long total = H->total[GF_E];
     const int col = tid % BN, part = tid / BN;
     const long long u0 = total * blockIdx.x / gridDim.x, u1 = total * (blockIdx.x + 1) / gridDim.x;
     if (u0 >= u1) return;
-    for (int idx = tid; idx < GF_MAXMP * (BN + 1); idx += 256) (&Ds[0][0])[idx] = (T)0;
+    for (int idx = tid; idx < BN * LDT; idx += 256) (&Dt[0][0])[idx] = (T)0;
+    // task of the current tile and the first unit past it
+    int ti = find_task(tasks, n, u0, GF_E);
+    long long tend = ti + 1 < n ? tasks[ti + 1].base[GF_E] : total;
     // beta-list entries of this thread's column for a tile (issued one tile ahead, so their
-    // global loads overlap the products of the current tile)
+    // global loads overlap the products of the current tile); entry i is e = part + i*PARTS
     uint32_t lcur[EMAX], lnxt[EMAX];
-    int ncur = 0, nnxt = 0;
-    auto fetch = [&](long long u, uint32_t (&dst)[EMAX], int &cnt) {
-        cnt = 0;
+    auto fetch = [&](long long u, uint32_t (&dst)[EMAX]) {
         if (u >= u1) return;
-        const GfTask &t = tasks[find_task(tasks, n, u, GF_E)];
+        const GfTask &t = tasks[u < tend ? ti : find_task(tasks, n, u, GF_E)];
         const long long N = t.g.na * t.g.nb, ncols = N * t.K;
-        const long long c = (u - t.base[GF_E]) * BN + col;
-        if (c >= ncols) return;
+        const long long c = std::min((u - t.base[GF_E]) * BN + col, ncols - 1);
         const long long Ib = (c % N) % t.g.nb;
         const auto lst = gptr(t.g.lb + Ib * t.g.Lb);
 #pragma unroll
         for (int i = 0; i < EMAX; i++) {
             const int e = part + i * PARTS;
-            if (e < t.g.Lb) { dst[i] = lst[e]; cnt = i + 1; }
+            if (e < t.g.Lb) dst[i] = lst[e];
         }
     };
-    fetch(u0, lcur, ncur);
+    fetch(u0, lcur);
     T a[2][KSMAX];
     int cur = -1;
     long long staged = -1;        // (task, k, Ja) of the row in xs, as task * 2^40 + k * N + Ja*Nb
     for (long long u = u0; u < u1; u++) {
-        const int ti = find_task(tasks, n, u, GF_E);
+        if (u >= tend) {
+            ti = find_task(tasks, n, u, GF_E);
+            tend = ti + 1 < n ? tasks[ti + 1].base[GF_E] : total;
+        }
         const GfTask &t = tasks[ti];
         const GemmFci &g = t.g;
-        const int MP = g.MP, KS = MP / 4, RT = MP / 16;
+        const int MP = g.MP, KS = MP / 4, RT = MP / 16, Lb = g.Lb;
+        const int nrt = (w < RT) + (w + 4 < RT);
         if (ti != cur) {
+            // A fragment of step ks: row rt*16 + l%16, column (l/16)*KS + ks
             const T *M2 = mpad<T>(g);
 #pragma unroll
             for (int i = 0; i < 2; i++) {
@@ -332,15 +414,15 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
 #pragma unroll
                 for (int ks = 0; ks < KSMAX; ks++)
                     a[i][ks] = (rt < RT && ks < KS)
-                                   ? M2[(long long)(rt * 16 + l % 16) * MP + ks * 4 + l / 16]
+                                   ? M2[(long long)(rt * 16 + l % 16) * MP + (l / 16) * KS + ks]
                                    : (T)0;
             }
             cur = ti;
         }
         const long long Nb = g.nb, N = g.na * g.nb, ncols = N * t.K;
         const long long c0 = (u - t.base[GF_E]) * BN;
-        // one vector row for the whole tile: stage it
         const long long clast = std::min(c0 + BN, ncols) - 1;
+        // one vector row for the whole tile: stage it
         const bool one_row = Nb <= GF_XROW && (c0 / Nb) == (clast / Nb);
         if (one_row) {
             const long long key = ((long long)ti << 40) + (c0 / Nb) * Nb;
@@ -352,44 +434,35 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
             }
         }
         __syncthreads();   // row staged; the previous tile's entries are cleared
-        const long long c = c0 + col;
-        if (c < ncols) {
+        {
             // all reads first (independent, one wait), then the scatter into the D tile
             T v[EMAX];
             if (one_row) {
 #pragma unroll
-                for (int i = 0; i < EMAX; i++) v[i] = i < ncur ? xs[lcur[i] & 0xFFFFu] : (T)0;
+                for (int i = 0; i < EMAX; i++)
+                    if (part + i * PARTS < Lb) v[i] = xs[lcur[i] & 0xFFFFu];
             } else {
+                const long long c = std::min(c0 + col, clast);
                 const long long k = c / N, Ja = (c - k * N) / Nb;
                 const auto xr = gptr(xin<T>(t, (int)k) + Ja * Nb);
 #pragma unroll
-                for (int i = 0; i < EMAX; i++) v[i] = i < ncur ? xr[lcur[i] & 0xFFFFu] : (T)0;
+                for (int i = 0; i < EMAX; i++)
+                    if (part + i * PARTS < Lb) v[i] = xr[lcur[i] & 0xFFFFu];
             }
 #pragma unroll
             for (int i = 0; i < EMAX; i++)
-                if (i < ncur) Ds[(lcur[i] >> 16) & 0xFFu][col] = (lcur[i] >> 31) ? -v[i] : v[i];
+                if (part + i * PARTS < Lb)
+                    Dt[col][(lcur[i] >> 16) & 0xFFu] = (lcur[i] >> 31) ? -v[i] : v[i];
         }
         __syncthreads();   // D tile complete
-        fetch(u + 1, lnxt, nnxt);
+        fetch(u + 1, lnxt);
         typename Mf<T>::acc acc[2][NCT];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int j = 0; j < NCT; j++) acc[i][j] = Mf<T>::zero();
-#pragma unroll
-        for (int ks = 0; ks < KSMAX; ks++) {
-            if (ks < KS) {
-                T b[NCT];
-#pragma unroll
-                for (int j = 0; j < NCT; j++) b[j] = Ds[ks * 4 + l / 16][j * 16 + l % 16];
-#pragma unroll
-                for (int i = 0; i < 2; i++)
-                    if (w + 4 * i < RT) {
-#pragma unroll
-                        for (int j = 0; j < NCT; j++) acc[i][j] = Mf<T>::mma(a[i][ks], b[j], acc[i][j]);
-                    }
-            }
-        }
+        if (nrt == 2) gE_mma_ks<T, BN, 2>(KS, acc, a, Dt, l);
+        else if (nrt == 1) gE_mma_ks<T, BN, 1>(KS, acc, a, Dt, l);
         const auto E = gptr_w((T *)t.E);
 #pragma unroll
         for (int i = 0; i < 2; i++) {
@@ -401,17 +474,16 @@ __global__ __launch_bounds__(256) void k_gE(const GfHeader *__restrict__ H,
                 for (int v = 0; v < 4; v++) {
                     const int P = rt * 16 + Mf<T>::row(l, v);
                     const long long cc = c0 + j * 16 + l % 16;
-                    if (P < g.m1 && cc < ncols) E[(long long)P * ncols + cc] = acc[i][j][v];
+                    if (P < g.m1 && cc < ncols) E[(long long)P * t.lde + cc] = acc[i][j][v];
                 }
         }
         __syncthreads();   // the products have read the D tile
-        // clear exactly the entries this thread wrote (the rest of Ds stays zero)
+        // clear exactly the entries this thread wrote (the rest of Dt stays zero)
 #pragma unroll
         for (int i = 0; i < EMAX; i++)
-            if (i < ncur) Ds[(lcur[i] >> 16) & 0xFFu][col] = (T)0;
+            if (part + i * PARTS < Lb) Dt[col][(lcur[i] >> 16) & 0xFFu] = (T)0;
 #pragma unroll
         for (int i = 0; i < EMAX; i++) lcur[i] = lnxt[i];
-        ncur = nnxt;
     }
 }
 
@@ -425,7 +497,7 @@ __global__ __launch_bounds__(256) void k_gga(const GfHeader *__restrict__ H,
     for (long long u = blockIdx.x; u < total; u += gridDim.x) {
         const GfTask &t = tasks[find_task(tasks, n, u, GF_GA)];
         const GemmFci &g = t.g;
-        const long long Nb = g.nb, N = g.na * g.nb, ncols = N * t.K;
+        const long long Nb = g.nb, N = g.na * g.nb, ncols = N * t.K, lde = t.lde;
         const long long idx = (u - t.base[GF_GA]) * 256 + threadIdx.x;
         if (idx >= ncols) continue;
         const long long k = idx / N, rem = idx - k * N;
@@ -440,14 +512,14 @@ __global__ __launch_bounds__(256) void k_gga(const GfHeader *__restrict__ H,
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const uint32_t wd = lst[e + q];
-                v[q] = Ek[(long long)((wd >> 16) & 0xFFu) * ncols + (long long)(wd & 0xFFFFu) * Nb];
+                v[q] = Ek[(long long)((wd >> 16) & 0xFFu) * lde + (long long)(wd & 0xFFFFu) * Nb];
             }
 #pragma unroll
             for (int q = 0; q < 8; q++) acc += (lst[e + q] >> 31) ? -v[q] : v[q];
         }
         for (; e < g.La; e++) {
             const uint32_t wd = lst[e];
-            const T v = Ek[(long long)((wd >> 16) & 0xFFu) * ncols + (long long)(wd & 0xFFFFu) * Nb];
+            const T v = Ek[(long long)((wd >> 16) & 0xFFu) * lde + (long long)(wd & 0xFFFFu) * Nb];
             acc += (wd >> 31) ? -v : v;
         }
         t.y[k * t.ldy + rem] += (double)acc;

@@ -7,7 +7,7 @@ Space: full CI 5 alpha + 5 beta electrons in 14 orbitals (4,008,004 determinants
 largest request the served stream draws), synthetic integrals seed 7.  Launches, each
 preceded by one warm-up: a 1 GiB device copy (k_copy, byte-counter calibration), then
 `--reps` FP64 sigmas (k_gE<double,32>, k_gss<double>, k_gga<double>) and `--reps` FP32 sigmas
-(k_gcv, k_gE<float,64>, k_gss<float>, k_gga<float>) of one vector, then the same with
+(k_gcv, k_gE<float,32>, k_gss<float>, k_gga<float>) of one vector, then the same with
 --nvec vectors.
 """
 import argparse

@@ -81,13 +81,17 @@ def test_batch_continuous_admission(J, orc, ctx):
         np.testing.assert_allclose(results[i], alone, rtol=0, atol=1e-8)
 
 
-def test_batch_pool_rounds_and_buffer_roots(J, orc, ctx):
+@pytest.mark.parametrize("tiles", [(0, 0), (16, 64)])
+def test_batch_pool_rounds_and_buffer_roots(J, orc, ctx, tiles):
     """A pool smaller than two vectors' intermediates forces one vector per sigma round; with
-    buffer roots the results still match the standalone solves and the dense spectrum."""
+    buffer roots the results still match the standalone solves and the dense spectrum.  Run
+    with the default E tile widths and with the other ones (FP64 16, FP32 64)."""
     specs = [(3, 3, 8, 0, 8, 0, 1, 2), (4, 4, 9, 0, 9, 0, 1, 1), (3, 2, 7, 0, 7, 0, 1, 2),
              (3, 3, 8, 2, 2, 2, 1, 1)]
     items = [_build(J, orc, ctx, spec, 300 + i) for i, spec in enumerate(specs)]
     b = J.Batch(ctx, 8, 16, pool_bytes=2 << 20)
+    if tiles[0]:
+        b.set_tiles(*tiles)
     slots = []
     for sp, h, w, nr in items:
         p = h.params(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),

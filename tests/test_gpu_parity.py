@@ -92,10 +92,14 @@ def test_sigma_all_methods(J, orc, ctx, spec):
 
 @pytest.mark.parametrize("spec,nvec", [((4, 4, 10, 0, 10, 0, 1), 1),   # 44,100 dets
                                        ((5, 4, 11, 0, 11, 0, 1), 3),   # open shell, 152,460
-                                       ((5, 5, 11, 0, 11, 0, 1), 2)])  # 213,444
+                                       ((5, 5, 11, 0, 11, 0, 1), 2),   # 213,444
+                                       ((1, 6, 15, 0, 15, 0, 1), 2)])  # 75,075
 def test_sigma_dense_string_larger(J, orc, ctx, spec, nvec):
     """The dense-string MFMA sigma on full-CI spaces whose string counts and orbital-pair
-    counts span several 64-row product tiles and many column tiles of the intermediate."""
+    counts span several 64-row product tiles and many column tiles of the intermediate.
+    1a6b/15: 5,005 beta strings (rows longer than the kernel's LDS row buffer, so the
+    gathers read the vector in device memory), 120 orbital pairs padded to 128 (the deepest
+    reduction) and 60-entry beta lists; 15 alpha strings."""
     gs, os_, m1, m2 = _spaces(J, orc, ctx, spec, seed=21)
     h = J.Hamiltonian(gs, m1, m2, J.DIRECT_TABLES)
     X = rand_vectors(gs.ndets, nvec, 8)
@@ -105,6 +109,28 @@ def test_sigma_dense_string_larger(J, orc, ctx, spec, nvec):
     np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * scale)
     y = h.sigma(X, method=J.SIGMA_GEMM32)
     np.testing.assert_allclose(y, ref, rtol=0, atol=2e-5 * scale)
+
+
+@pytest.mark.parametrize("spec,nvec", [((3, 3, 8, 0, 8, 0, 1), 3),     # 56 beta strings: tiles wrap rows
+                                       ((5, 4, 11, 0, 11, 0, 1), 2)])  # 152,460
+def test_sigma_dense_tile_widths(J, orc, ctx, spec, nvec):
+    """Every E-kernel column-tile width (FP64 16 and 32, FP32 32 and 64) against the oracle;
+    invalid widths are refused."""
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec, seed=5)
+    h = J.Hamiltonian(gs, m1, m2, J.DIRECT_TABLES)
+    X = rand_vectors(gs.ndets, nvec, 9)
+    ref = os_.sigma(m1, m2, X)
+    scale = np.abs(ref).max()
+    for bn64, bn32 in ((16, 32), (32, 64)):
+        assert h.gemm_tiles(bn64, bn32) == (bn64, bn32)
+        y = h.sigma(X, method=J.SIGMA_GEMM64)
+        np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * scale, err_msg="bn64=%d" % bn64)
+        y = h.sigma(X, method=J.SIGMA_GEMM32)
+        np.testing.assert_allclose(y, ref, rtol=0, atol=2e-5 * scale, err_msg="bn32=%d" % bn32)
+    with pytest.raises(J.JciError):
+        h.gemm_tiles(64, 0)
+    with pytest.raises(J.JciError):
+        h.gemm_tiles(0, 16)
 
 
 def test_pair_counts_fci(J, orc, ctx):
