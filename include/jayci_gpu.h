@@ -161,6 +161,8 @@ typedef struct jci_ham jci_ham;
 #define JCI_GUESS_BLOCK 1   /* leading refdim block (source/initguess_sbd.c:24) */
 #define JCI_GUESS_USER 2    /* caller vectors (source/initguess_roldv.c:20) */
 #define JCI_GUESS_DIAG 3    /* block of the refdim lowest diagonal elements */
+#define JCI_GUESS_UNIT 4    /* unit vectors on determinants 0..krymin-1 (the MPI driver's
+                               prediagr 2, mpi_source/pdavidson.c:418-431) */
 
 #define JCI_MAXROOTS 8
 
@@ -232,6 +234,29 @@ void jci_solve_defaults(jci_solve_params *p);
  * (vector-major); guess (JCI_GUESS_USER) holds krymin vectors of N */
 int jci_solve(jci_ham *h, const jci_solve_params *p, const double *guess, double *evals,
               double *evecs, jci_solve_stats *stats);
+
+/* The MPI driver's binary vector files: int32 dim, int32 len, then dim vectors of len
+ * doubles (native byte order, no record markers).
+ * jci_write_vectors_ufmt replaces mpi_source/mpi_utilities.c:353
+ * (print_gavectors2file_dbl_ufmt) for host rows vecs[0..dim-1]: 0 ok, 2 cannot open,
+ * 6 write error.  jci_read_vectors_ufmt replaces mpi_source/mpi_utilities.c:426
+ * (read_gavectorsfile_dbl_ufmt): 0 ok, 2 cannot open, 3 / 4 short header, 5 the file's
+ * dim or len differs from the arguments, 6 short data (the reference's codes 2-5; it
+ * reports short data by a nonzero position instead). */
+int jci_write_vectors_ufmt(const char *fname, int len, int dim, double *const *vecs);
+int jci_read_vectors_ufmt(const char *fname, int len, int dim, double **vecs);
+
+/* replaces the Davidson step of the MPI driver, mpi_source/pdavidson.c:39 (pdavidson),
+ * for a handle: prediagr 1 guess from the leading refdim block, 2 unit vectors on the
+ * first krymin determinants (krymin <= refdim), 3 krymin vectors read from
+ * dir/ci.restart.  Writes dir/civec (the nroots Ritz vectors) and, when not converged,
+ * dir/ci.restart (the lowest krymin Ritz vectors) and dir/hvvec (nroots vectors of zeros:
+ * the reference clears its sigma block before its loop ends); dir NULL = working
+ * directory.  evals (may be NULL): nroots energies + p->totfrze.  Returns 0 converged,
+ * 1 not converged (files written; the reference returns 0 there), a file code above,
+ * or < 0 on error. */
+int jci_pdavidson(jci_ham *h, const jci_solve_params *p, int prediagr, const char *dir,
+                  double *evals, jci_solve_stats *stats);
 
 /* G = V^T W for tall-skinny blocks on the FP64 matrix cores (the Davidson projection
  * kernel): V is m vectors of length n, W is w vectors (vector-major, host memory),

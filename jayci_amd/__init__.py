@@ -21,7 +21,7 @@ PAIRS_FP64, PAIRS_FP32, PAIRS_UPPER, DIRECT_TABLES = 1, 2, 4, 8
  SIGMA_FAST64, SIGMA_FAST32, SIGMA_FCI64, SIGMA_FCI32, SIGMA_GEMM64, SIGMA_GEMM32) = range(12)
 NMETHODS = 12
 PREC_FP64, PREC_MIXED = 0, 1
-GUESS_BLOCK, GUESS_USER, GUESS_DIAG = 1, 2, 3
+GUESS_BLOCK, GUESS_USER, GUESS_DIAG, GUESS_UNIT = 1, 2, 3, 4
 MAXROOTS = 8
 SYNTH_ENUC = 10.0
 
@@ -100,6 +100,10 @@ _sig("jci_batch_result", _i, [_P, _i, ctypes.POINTER(_i), _P, _P, ctypes.POINTER
 _sig("jci_batch_release", _i, [_P, _i])
 _sig("jci_batch_sigma_seconds", _d, [_P])
 _sig("jci_batch_phase_seconds", _i, [_P, _P])
+_sig("jci_write_vectors_ufmt", _i, [ctypes.c_char_p, _i, _i, _P])
+_sig("jci_read_vectors_ufmt", _i, [ctypes.c_char_p, _i, _i, _P])
+_sig("jci_pdavidson", _i, [_P, ctypes.POINTER(SolveParams), _i, ctypes.c_char_p, _P,
+                           ctypes.POINTER(SolveStats)])
 
 
 class JciError(RuntimeError):
@@ -401,6 +405,52 @@ class Hamiltonian:
                  "rnorm": [st.rnorm[i] for i in range(nroots)], "seconds": st.seconds,
                  "sigma_seconds": st.sigma_seconds}
         return ev, vec, stats
+
+
+    def pdavidson(self, prediagr=1, directory=None, nroots=1, restol=1e-6, maxiter=200,
+                  krymin=None, krymax=None, refdim=64, precision=PREC_FP64, totfrze=0.0,
+                  verbose=False, nbuffer=0, etol=0.0):
+        """The MPI driver's Davidson step (jci_pdavidson): prediagr 1 block guess, 2 unit
+        vectors, 3 directory/ci.restart; writes civec (and ci.restart, hvvec when not
+        converged) into `directory` (None = working directory).  Returns (energies, stats)
+        with stats["rc"] 0 converged or 1 not converged."""
+        p = self.params(nroots=nroots, restol=restol, maxiter=maxiter, krymin=krymin,
+                        krymax=krymax, refdim=refdim, precision=precision, totfrze=totfrze,
+                        verbose=verbose, nbuffer=nbuffer, etol=etol)
+        ev = np.zeros(nroots)
+        st = SolveStats()
+        d = None if directory is None else os.fsencode(directory)
+        rc = _L.jci_pdavidson(self._h, ctypes.byref(p), prediagr, d, _p(ev), ctypes.byref(st))
+        if rc not in (0, 1):
+            raise JciError("jci_pdavidson failed (%d): %s" % (rc, last_error()))
+        return ev, {"rc": rc, "iterations": st.iterations, "converged": bool(st.converged),
+                    "rnorm": [st.rnorm[i] for i in range(nroots)]}
+
+
+def _rows(a):
+    """ctypes array of row pointers into a C-contiguous float64 matrix."""
+    return (_P * a.shape[0])(*[a[i:].ctypes.data_as(_P) for i in range(a.shape[0])])
+
+
+def write_vectors_ufmt(fname, vecs):
+    """Write (dim, len) vectors in the MPI driver's binary layout (jci_write_vectors_ufmt)."""
+    v = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+    rc = _L.jci_write_vectors_ufmt(os.fsencode(fname), v.shape[1], v.shape[0],
+                                   ctypes.cast(_rows(v), _P))
+    if rc:
+        raise JciError("jci_write_vectors_ufmt failed (%d): %s" % (rc, last_error()))
+
+
+def read_vectors_ufmt(fname, length, dim):
+    """Read dim vectors of `length` doubles (jci_read_vectors_ufmt).  Raises JciError with
+    the reference's code in .code on failure."""
+    v = np.zeros((dim, length))
+    rc = _L.jci_read_vectors_ufmt(os.fsencode(fname), length, dim, ctypes.cast(_rows(v), _P))
+    if rc:
+        e = JciError("jci_read_vectors_ufmt failed (%d): %s" % (rc, last_error()))
+        e.code = rc
+        raise e
+    return v
 
 
 def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,

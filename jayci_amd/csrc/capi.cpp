@@ -19,6 +19,7 @@
 #include <math.h>
 #include <mutex>
 #include <map>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -537,7 +538,7 @@ JCI_API int dvdalg(struct det *dlist, int ndets, double *moints1, double *moints
     so.resolve = list_resolve;
     std::vector<double> ev(nroots), vec((size_t)nroots * ndets);
     jci_solve_stats st;
-    rc = jci_solve_core(&so, &p, guess.empty() ? nullptr : guess.data(), ev.data(), vec.data(), &st);
+    rc = jci_solve_core(&so, &p, guess.empty() ? nullptr : guess.data(), ev.data(), vec.data(), &st, 0);
     hipFree(d_pre);
     if (rc < 0) { report(rc, "dvdalg"); return rc; }
     if (rc == 0) fprintf(stdout, "\n ** CI CONVERGED! **\n");
@@ -548,6 +549,99 @@ JCI_API int dvdalg(struct det *dlist, int ndets, double *moints1, double *moints
     }
     // like the reference, running out of iterations is not an error code
     return 0;
+}
+
+// ---- the MPI driver's binary vector files and its Davidson step --------------------------------
+// Layout of mpi_source/mpi_utilities.c:353-415 (writer) and :426-531 (reader): int32 dim,
+// int32 len, then dim vectors of len doubles, native byte order, no record markers.
+JCI_API int jci_write_vectors_ufmt(const char *fname, int len, int dim, double *const *vecs)
+{
+    if (len < 0 || dim < 0 || (dim > 0 && !vecs)) { jci_set_error("write_vectors_ufmt: bad arguments"); return -1; }
+    FILE *f = fopen(fname, "wb");
+    if (!f) { jci_set_error("write_vectors_ufmt: cannot open %s", fname); return 2; }
+    const int32_t hdr[2] = {dim, len};
+    int rc = fwrite(hdr, sizeof(int32_t), 2, f) == 2 ? 0 : 6;
+    for (int i = 0; i < dim && rc == 0; i++)
+        if (fwrite(vecs[i], sizeof(double), (size_t)len, f) != (size_t)len) rc = 6;
+    if (fclose(f) != 0 && rc == 0) rc = 6;
+    if (rc) jci_set_error("write_vectors_ufmt: write error on %s", fname);
+    return rc;
+}
+
+JCI_API int jci_read_vectors_ufmt(const char *fname, int len, int dim, double **vecs)
+{
+    if (len < 0 || dim < 0 || (dim > 0 && !vecs)) { jci_set_error("read_vectors_ufmt: bad arguments"); return -1; }
+    FILE *f = fopen(fname, "rb");
+    if (!f) { jci_set_error("read_vectors_ufmt: cannot open %s", fname); return 2; }
+    int32_t fdim = 0, flen = 0;
+    int rc = 0;
+    if (fread(&fdim, sizeof(int32_t), 1, f) != 1) rc = 3;
+    else if (fread(&flen, sizeof(int32_t), 1, f) != 1) rc = 4;
+    else if (fdim != dim || flen != len) rc = 5;
+    for (int i = 0; i < dim && rc == 0; i++)
+        if (fread(vecs[i], sizeof(double), (size_t)len, f) != (size_t)len) rc = 6;
+    fclose(f);
+    if (rc == 5)
+        jci_set_error("read_vectors_ufmt: %s holds %d x %d, expected %d x %d", fname, fdim, flen, dim, len);
+    else if (rc)
+        jci_set_error("read_vectors_ufmt: %s is short (code %d)", fname, rc);
+    return rc;
+}
+
+static std::string in_dir(const char *dir, const char *name)
+{
+    if (!dir || !*dir) return name;
+    std::string s(dir);
+    if (s.back() != '/') s += '/';
+    return s + name;
+}
+
+// mpi_source/pdavidson.c:39-356 on a handle: guess by prediagr (:183, :385-449), the solve,
+// then civec and, unconverged, ci.restart and hvvec (:343-347)
+JCI_API int jci_pdavidson(jci_ham *h, const jci_solve_params *p, int prediagr, const char *dir,
+                          double *evals, jci_solve_stats *stats)
+{
+    if (!h || !p) { jci_set_error("pdavidson: null argument"); return -1; }
+    const long long N = jci_ham_ndets(h);
+    if (N > INT32_MAX) { jci_set_error("pdavidson: %lld determinants exceed the file format", N); return -1; }
+    if (p->nroots < 1 || p->krymin < p->nroots) { jci_set_error("pdavidson: need 1 <= nroots <= krymin"); return -1; }
+    jci_solve_params q = *p;
+    std::vector<double> guess;
+    if (prediagr == 1) {
+        q.guess = JCI_GUESS_BLOCK;
+    } else if (prediagr == 2) {
+        if (p->krymin > p->refdim) { jci_set_error("pdavidson: unit-vector guess needs krymin <= refdim"); return -1; }
+        q.guess = JCI_GUESS_UNIT;
+    } else if (prediagr == 3) {
+        guess.resize((size_t)p->krymin * N);
+        std::vector<double *> rows(p->krymin);
+        for (int r = 0; r < p->krymin; r++) rows[r] = guess.data() + (size_t)r * N;
+        int rc = jci_read_vectors_ufmt(in_dir(dir, "ci.restart").c_str(), (int)N, p->krymin, rows.data());
+        if (rc) return rc;
+        q.guess = JCI_GUESS_USER;
+    } else {
+        jci_set_error("pdavidson: prediagr must be 1, 2 or 3");
+        return -1;
+    }
+    const int nout = std::max(p->nroots, p->krymin);
+    std::vector<double> ev(p->nroots), vec((size_t)nout * N);
+    jci_solve_stats st;
+    int rc = jci_solve_nout(h, &q, guess.empty() ? nullptr : guess.data(), ev.data(), vec.data(), &st, nout);
+    if (stats) *stats = st;
+    if (rc < 0) return rc;
+    if (evals) for (int r = 0; r < p->nroots; r++) evals[r] = ev[r];
+    std::vector<double *> rows(nout);
+    for (int r = 0; r < nout; r++) rows[r] = vec.data() + (size_t)r * N;
+    int wrc = jci_write_vectors_ufmt(in_dir(dir, "civec").c_str(), (int)N, p->nroots, rows.data());
+    if (!wrc && rc == 1) {
+        wrc = jci_write_vectors_ufmt(in_dir(dir, "ci.restart").c_str(), (int)N, p->krymin, rows.data());
+        // the reference clears its sigma block before the loop ends, so hvvec holds zeros
+        std::vector<double> zero(N, 0.0);
+        std::vector<double *> zrows(p->nroots, zero.data());
+        if (!wrc) wrc = jci_write_vectors_ufmt(in_dir(dir, "hvvec").c_str(), (int)N, p->nroots, zrows.data());
+    }
+    if (wrc) return wrc;
+    return rc;
 }
 
 // ---- output files -------------------------------------------------------------------------------

@@ -1132,7 +1132,7 @@ static int guess_block(Work &w, const jci_solve_params *p, int nguess)
 }
 
 int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, double *evals,
-                   double *evecs, jci_solve_stats *stats)
+                   double *evecs, jci_solve_stats *stats, int nout)
 {
     double t_start = now_s();
     jci_ctx *c = op->ctx;
@@ -1185,6 +1185,14 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
                 m += kept;
             }
         }
+    } else if (p->guess == JCI_GUESS_UNIT) {
+        // unit vectors on determinants 0..krymin-1 (mpi_source/pdavidson.c:418-431)
+        if (hipMemsetAsync(w.V, 0, vb * krymin, c->stream) != hipSuccess) { rc = -2; goto done; }
+        static const double one = 1.0;
+        for (int i = 0; i < krymin; i++)
+            if (hipMemcpyAsync(w.V + (size_t)i * w.ld + i, &one, sizeof(double), hipMemcpyHostToDevice,
+                               c->stream) != hipSuccess) { rc = -2; goto done; }
+        m = krymin;
     } else {
         if ((rc = guess_block(w, p, krymin))) goto done;
         m = krymin;
@@ -1273,6 +1281,9 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             if ((rc = jk_rotate_inplace(m, kc, N, w.V, w.ld, dY, c->stream))) goto done;
             if ((rc = jk_rotate_inplace(m, kc, N, w.S, w.ld, dY, c->stream))) goto done;
             m = kc;
+            // the collapsed basis vectors are the Ritz vectors themselves
+            Yf.assign((size_t)m * m, 0.0);
+            for (int i = 0; i < m; i++) Yf[(size_t)i * m + i] = 1.0;
         }
         int kept = 0;
         if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;
@@ -1285,15 +1296,23 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     }
     for (int r = 0; r < nroots; r++) evals[r] = theta[r] + p->totfrze;
     if (evecs) {
-        std::vector<double> Yc((size_t)m * nroots);
-        for (int i = 0; i < m; i++)
-            for (int r = 0; r < nroots; r++) Yc[(size_t)i * nroots + r] = Yf[(size_t)r * m + i];
-        if ((rc = upload_small(w, Yc.data(), m * nroots, &dY))) goto done;
-        if ((rc = jk_combine(m, nroots, N, w.V, w.ld, dY, nroots, w.T, w.ld, 0.0, c->stream))) goto done;
-        for (int r = 0; r < nroots; r++)
-            if (hipMemcpyAsync(evecs + (size_t)r * N, w.T + (size_t)r * w.ld, sizeof(double) * N,
-                               hipMemcpyDeviceToHost, c->stream) != hipSuccess) { rc = -2; goto done; }
-        if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = -2; goto done; }
+        // the lowest nv Ritz vectors, JCI_MAXK at a time through T; rows past the basis
+        // size (nout > m) are zero
+        const int nv = std::max(nroots, nout);
+        for (int r0 = 0; r0 < nv; r0 += JCI_MAXK) {
+            const int kb = std::min(JCI_MAXK, std::min(nv, m) - r0);
+            if (kb <= 0) break;
+            std::vector<double> Yc((size_t)m * kb);
+            for (int i = 0; i < m; i++)
+                for (int r = 0; r < kb; r++) Yc[(size_t)i * kb + r] = Yf[(size_t)(r0 + r) * m + i];
+            if ((rc = upload_small(w, Yc.data(), m * kb, &dY))) goto done;
+            if ((rc = jk_combine(m, kb, N, w.V, w.ld, dY, kb, w.T, w.ld, 0.0, c->stream))) goto done;
+            for (int r = 0; r < kb; r++)
+                if (hipMemcpyAsync(evecs + (size_t)(r0 + r) * N, w.T + (size_t)r * w.ld, sizeof(double) * N,
+                                   hipMemcpyDeviceToHost, c->stream) != hipSuccess) { rc = -2; goto done; }
+            if (hipStreamSynchronize(c->stream) != hipSuccess) { rc = -2; goto done; }
+        }
+        for (int r = std::min(nv, m); r < nv; r++) memset(evecs + (size_t)r * N, 0, sizeof(double) * N);
     }
     if (!converged && rc == 0) {
         jci_set_error("solve: not converged in %d iterations (max|r| above %.3e)", p->maxiter, p->restol);
@@ -1358,7 +1377,25 @@ JCI_API int jci_solve(jci_ham *h, const jci_solve_params *p, const double *guess
     op.sigma_cols = ham_sigma_cols;
     op.masks = ham_masks;
     op.resolve = ham_resolve;
-    return jci_solve_core(&op, p, guess, evals, evecs, stats);
+    return jci_solve_core(&op, p, guess, evals, evecs, stats, 0);
+}
+
+// the lowest max(nroots, nout) Ritz vectors into evecs (the MPI driver's restart file)
+int jci_solve_nout(jci_ham *h, const jci_solve_params *p, const double *guess, double *evals,
+                   double *evecs, jci_solve_stats *stats, int nout)
+{
+    SolveOp op{};
+    op.self = h;
+    op.ctx = h->sp->ctx;
+    op.N = h->sp->h.ndets;
+    op.d_hd = h->d_hd;
+    op.d_m1 = h->d_m1;
+    op.d_m2 = h->d_m2;
+    op.sigma = ham_sigma;
+    op.sigma_cols = ham_sigma_cols;
+    op.masks = ham_masks;
+    op.resolve = ham_resolve;
+    return jci_solve_core(&op, p, guess, evals, evecs, stats, nout);
 }
 
 // ---- accessors for the batched engine ---------------------------------------------------------

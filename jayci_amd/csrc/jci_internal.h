@@ -80,7 +80,11 @@ struct SolveOp {
 };
 #include "../../include/jayci_gpu.h"
 int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, double *evals,
-                   double *evecs, jci_solve_stats *stats);
+                   double *evecs, jci_solve_stats *stats, int nout);
+long long jci_ham_ndets(jci_ham *h);
+// jci_solve returning the lowest max(nroots, nout) Ritz vectors
+int jci_solve_nout(jci_ham *h, const jci_solve_params *p, const double *guess, double *evals,
+                   double *evecs, jci_solve_stats *stats, int nout);
 
 // ---- kernel launchers (kernels.hip) ------------------------------------------------
 int jk_gen_strings(const SpaceDesc *d_desc, const SpaceDesc &h, int spin,

@@ -253,3 +253,46 @@ def test_mrcisd_16e60o_sampled_rows_bitexact(J, orc, ctx):
         assert np.array_equal(ph, oph), "phases differ in row %d" % I
         assert np.array_equal(dg, odg), "degrees differ in row %d" % I
         np.testing.assert_allclose(v, ov, rtol=1e-12, atol=1e-13)
+
+
+def test_pdavidson_guess_modes_and_restart_files(J, orc, ctx, tmp_path):
+    """The MPI driver's Davidson step: prediagr 1 (block guess), 2 (unit vectors) and 3
+    (ci.restart) reach the dense eigenvalues; civec holds the Ritz vectors; an unconverged
+    run writes ci.restart (krymin vectors) and hvvec (zeros, as the reference does), and a
+    run restarted from that file converges to the same energies."""
+    spec = (3, 3, 8, 0, 8, 0, 1)
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec, seed=6)
+    N = gs.ndets
+    H = os_.dense_h(m1, m2)
+    w = np.linalg.eigh(H)[0]
+    h = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64)
+    kw = dict(nroots=2, restol=1e-7, krymin=4, krymax=12, refdim=40)
+    for prediagr in (1, 2):
+        d = tmp_path / ("p%d" % prediagr)
+        d.mkdir()
+        ev, st = h.pdavidson(prediagr, str(d), **kw)
+        assert st["rc"] == 0
+        np.testing.assert_allclose(ev, w[:2], rtol=0, atol=1e-8)
+        vec = J.read_vectors_ufmt(str(d / "civec"), N, 2)
+        for r in range(2):
+            assert np.linalg.norm(H @ vec[r] - ev[r] * vec[r]) < 1e-6
+        assert not (d / "ci.restart").exists() and not (d / "hvvec").exists()
+    # unconverged: restart files written
+    d = tmp_path / "short"
+    d.mkdir()
+    ev0, st = h.pdavidson(1, str(d), nroots=2, restol=1e-7, krymin=4, krymax=8, refdim=40,
+                          maxiter=2)
+    assert st["rc"] == 1
+    rs = J.read_vectors_ufmt(str(d / "ci.restart"), N, 4)
+    np.testing.assert_allclose(rs @ rs.T, np.eye(4), atol=1e-10)
+    assert not J.read_vectors_ufmt(str(d / "hvvec"), N, 2).any()
+    # prediagr 3 continues from ci.restart
+    ev3, st3 = h.pdavidson(3, str(d), **kw)
+    assert st3["rc"] == 0
+    np.testing.assert_allclose(ev3, w[:2], rtol=0, atol=1e-8)
+    ev1, st1 = h.pdavidson(1, str(tmp_path / "p1"), **kw)
+    assert st3["iterations"] < st1["iterations"]
+    # a restart file of other dimensions is refused with the reader's code
+    J.write_vectors_ufmt(str(d / "ci.restart"), np.zeros((3, N)))
+    with pytest.raises(J.JciError):
+        h.pdavidson(3, str(d), **kw)
