@@ -119,6 +119,19 @@ int read_civectors(int aelec, int belec, int orbs, double **civec, int ndets, in
 void write_wavefunction(struct det *detlist, int ndets, int nroots, double **civec,
                         double *cival, int orbs, int ninto, int elec);
 
+/* replaces source/ioutil.c:427 (readmointegrals), which calls the Fortran readmoints
+ * (source/readmoints.f90:21): reads the SIFS integral file (8-byte integers, 4-byte record
+ * markers, fsplit 1) into moints1 (H1, or the three dipole arrays for itype 3) and
+ * moints2, *nuc_rep = energy(1), *fcenergy = frozen-core energy.  The reference opens
+ * "moints" and only prints moflname; here a non-empty moflname names the file.  On a bad
+ * file it prints the reference's message and ends the program, as the reference's STOP. */
+void readmointegrals(double *moints1, double *moints2, int itype, int orbitals, char *moflname,
+                     int m1len, int m2len, double *nuc_rep, double *fcenergy);
+/* the same reader returning instead of ending the program: 0, or -1 with the message in
+ * jci_last_error(); en[0] nuclear repulsion, en[1] frozen-core energy */
+int jci_readmoints(const char *path, int type1n, int orbitals, double *moints1, long long m1len,
+                   double *moints2, long long m2len, double *en);
+
 /* replace source/ioutil.c:332 (readgeninput) and :280 (readdaiinput): namelists of
  * jayci.in in the working directory (source/readnamelist.f90:38-41, defaults :79-85);
  * restol is parsed at full precision */

@@ -101,6 +101,7 @@ _sig("jci_batch_release", _i, [_P, _i])
 _sig("jci_batch_sigma_seconds", _d, [_P])
 _sig("jci_batch_phase_seconds", _i, [_P, _P])
 _sig("jci_write_vectors_ufmt", _i, [ctypes.c_char_p, _i, _i, _P])
+_sig("jci_readmoints", _i, [ctypes.c_char_p, _i, _i, _P, _ll, _P, _ll, _P])
 _sig("jci_read_vectors_ufmt", _i, [ctypes.c_char_p, _i, _i, _P])
 _sig("jci_pdavidson", _i, [_P, ctypes.POINTER(SolveParams), _i, ctypes.c_char_p, _P,
                            ctypes.POINTER(SolveStats)])
@@ -451,6 +452,20 @@ def read_vectors_ufmt(fname, length, dim):
         e.code = rc
         raise e
     return v
+
+
+def read_moints(path, norbs, type1n=1):
+    """Read a SIFS integral file (jci_readmoints): returns (moints1, moints2, nuc_rep,
+    frozen_core) with moints1 of m1_len(norbs) (3 arrays for type1n 3) and moints2 of
+    m2_len(norbs), packed as the reference's index1e / index2e."""
+    m1 = np.zeros(m1_len(norbs) * (3 if type1n == 3 else 1))
+    m2 = np.zeros(m2_len(norbs))
+    en = np.zeros(2)
+    rc = _L.jci_readmoints(os.fsencode(path), type1n, norbs, _p(m1), m1_len(norbs), _p(m2),
+                           m2_len(norbs), _p(en))
+    if rc:
+        raise JciError("jci_readmoints failed (%d): %s" % (rc, last_error()))
+    return m1, m2, float(en[0]), float(en[1])
 
 
 def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libjayci_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["kernels.hip", "sigma_fast.hip", "sigma_gemm.hip", "batch_kernels.hip"]
-CXX_SOURCES = ["space.cpp", "jci.cpp", "capi.cpp", "batch.cpp", "symeig.cpp"]
+CXX_SOURCES = ["space.cpp", "jci.cpp", "capi.cpp", "batch.cpp", "symeig.cpp", "sifs.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden", "-Wno-unused-result", "-Wno-unused-value", "-I" + CSRC,
           "-I" + os.path.join(ROOT, "include")]

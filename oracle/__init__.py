@@ -19,10 +19,13 @@ _lib = None
 
 
 def build(ref=False):
-    """Compile liboracle.so (and oracle/_ref/libjayci_ref.so when ref=True)."""
+    """Compile liboracle.so (and, when ref=True, oracle/_ref/libjayci_ref.so and the SIFS
+    fixture writer oracle/_ref/sifs_ref)."""
     subprocess.check_call(["make", "-s", "-C", HERE, "all"])
     if ref:
         subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+        subprocess.check_call(["sh", os.path.join(HERE, "sifs", "build.sh")],
+                              stdout=subprocess.DEVNULL)
 
 
 def lib():
