@@ -442,7 +442,7 @@ static int run_sigma_gf(jci_batch *b, const std::vector<int> &ids, int prec)
     const size_t esz = prec == 64 ? 8 : 4;
     size_t need = 0;
     for (int i : ids)   // one vector of the largest instance, with the row padding
-        need = std::max(need, (size_t)b->in[i].g.m1 * (size_t)(b->in[i].N + GF_EALIGN) * esz);
+        need = std::max(need, (size_t)b->in[i].g.La * (size_t)(b->in[i].N + GF_EALIGN) * esz);
     if (!b->epool || b->epool_bytes < need) {
         JCI_CHECK(hipStreamSynchronize(c->stream));
         hipFree(b->epool);
@@ -464,9 +464,9 @@ static int run_sigma_gf(jci_batch *b, const std::vector<int> &ids, int prec)
         for (int i : ids) {
             BInst &s = b->in[i];
             if (s.sig_count <= 0) continue;
-            // E of k vectors: m1 rows of jk_gf_lde(k N) elements
-            const size_t per = (size_t)s.g.m1 * s.N * esz;
-            const size_t pad = (size_t)s.g.m1 * GF_EALIGN * esz;
+            // E of k vectors: La rows of jk_gf_lde(k N) elements
+            const size_t per = (size_t)s.g.La * s.N * esz;
+            const size_t pad = (size_t)s.g.La * GF_EALIGN * esz;
             const size_t avail = b->epool_bytes - used;
             const size_t room = avail > pad ? (avail - pad) / per : 0;
             const int k = (int)std::min<size_t>(std::min<size_t>(s.sig_count, JCI_MAXK), room);
@@ -485,7 +485,7 @@ static int run_sigma_gf(jci_batch *b, const std::vector<int> &ids, int prec)
             long long u[4];
             jk_gf_units(prec, prec == 64 ? b->bn64 : b->bn32, t, u);
             for (int q = 0; q < 4; q++) { t.base[q] = base[q]; base[q] += u[q]; }
-            used += ((size_t)s.g.m1 * (size_t)t.lde * esz + 255) / 256 * 256;
+            used += ((size_t)s.g.La * (size_t)t.lde * esz + 255) / 256 * 256;
             who.push_back(i);
             kk.push_back(k);
             n++;

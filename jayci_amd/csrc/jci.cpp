@@ -368,8 +368,27 @@ static int build_gf(jci_ham *h)
     RCHECK(jk_gf_m2pad(m1, MP, h->d_m2sq, h->d_gfM64, h->d_gfM32, c->stream));
     JCI_CHECK(hipMalloc(&h->d_gfla, sizeof(uint32_t) * na * La));
     JCI_CHECK(hipMalloc(&h->d_gflb, sizeof(uint32_t) * nb * Lb));
-    RCHECK(jk_gf_lists(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], La, h->d_gfla, c->stream));
     RCHECK(jk_gf_lists(nb, s->d_mask[1], h->d_off1[1], h->d_e1[1], Lb, h->d_gflb, c->stream));
+    {
+        // alpha lists with the E row of each entry (the pair's position in the target's list)
+        uint32_t *raw = nullptr;
+        int *d_missing = nullptr, missing = 0;
+        JCI_CHECK(hipMalloc(&raw, sizeof(uint32_t) * na * La));
+        JCI_CHECK(hipMalloc(&d_missing, sizeof(int)));
+        JCI_CHECK(hipMemsetAsync(d_missing, 0, sizeof(int), c->stream));
+        int rc = jk_gf_lists(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], La, raw, c->stream);
+        if (!rc) rc = jk_gf_rank(na, La, raw, h->d_gfla, d_missing, c->stream);
+        if (!rc && hipMemcpyAsync(&missing, d_missing, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = -2;
+        if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = -2;
+        hipFree(raw);
+        hipFree(d_missing);
+        if (rc) return rc;
+        if (missing) {
+            jci_set_error("gf: %d alpha-list entries without a row in their target's list", missing);
+            return -1;
+        }
+    }
     JCI_CHECK(hipStreamSynchronize(c->stream));
     GemmFci &g = h->gf;
     g.na = na;
@@ -652,11 +671,11 @@ static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, d
         JCI_CHECK(hipHostMalloc(&h->h_gfh, sizeof(GfHeader) * JCI_MAXK, hipHostMallocDefault));
         memset(h->gf_valid, 0, sizeof(h->gf_valid));
     }
-    // vectors per pass: the intermediate E (m1 x k x N) stays within 8 GB
-    const int Kc = (int)std::max(1.0, std::min((double)K, 8e9 / ((double)h->gf.m1 * N * esz)));
+    // vectors per pass: the intermediate E (La x k x N) stays within 8 GB
+    const int Kc = (int)std::max(1.0, std::min((double)K, 8e9 / ((double)h->gf.La * N * esz)));
     for (int b = 0, ci = 0; b < K; b += Kc, ci++) {
         const int k = std::min(Kc, K - b);
-        RCHECK(grow(&h->d_gfE, &h->gfE_bytes, (size_t)h->gf.m1 * jk_gf_lde((long long)k * N) * esz,
+        RCHECK(grow(&h->d_gfE, &h->gfE_bytes, (size_t)h->gf.La * jk_gf_lde((long long)k * N) * esz,
                     "the intermediate"));
         if (prec == 32)
             RCHECK(grow((void **)&h->d_gfX32, &h->gfX32_len, sizeof(float) * (size_t)k * N,

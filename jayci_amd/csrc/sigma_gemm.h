@@ -17,6 +17,13 @@
 //   E[P][k,Ja,Ib] = sum_Q (P|Q) D[Q][k,Ja,Ib]                            (MFMA, D in LDS)
 //   sigma_k[Ia][Ib] += sum_{(Ja,P,sign) in alpha list of Ia} sign * E[P][k,Ja,Ib]
 //
+// E[P][.,Ja,.] is read only for the pairs P that connect Ja to some Ia: the pairs of the
+// replacement list of Ja itself (a single i -> a of Ia lands on Ja exactly when a -> i
+// is a single of Ja; the diagonal pairs are those of Ja's occupied orbitals).  So E is
+// kept as La rows per alpha string: row p of string Ja is the pair of entry p of Ja's
+// alpha list, and each alpha-list entry carries, in bits 24..30, the position of its pair
+// in the list of its target string.
+//
 // Matrix elements are those of the Slater-Condon rules in slater.h (the corrected
 // same-spin doubles of mpi_source/action_util.c:215-240).
 #pragma once
@@ -26,7 +33,8 @@
 #define GF_MAXSTR 65535       // strings per spin addressable by a 16-bit list entry
 #define GF_MAXLIST 64         // replacement-list entries per string: b(n-b+1) <= 64 for n <= 15
 
-// list entry: bits 0..15 target string, 16..23 orbital pair, bit 31 sign
+// list entry: bits 0..15 target string, 16..23 orbital pair, bit 31 sign; alpha lists also
+// bits 24..30: position of the pair in the target string's list (the E row)
 struct GemmFci {
     long long na, nb;            // alpha / beta strings
     int m1, MP;                  // orbital pairs, padded to a multiple of 16
@@ -50,7 +58,7 @@ struct GfTask {
     float *x32;                  // FP32 copies [K][N] (FP32 launches)
     double *y;                   // output columns, vector k at y + k*ldy (overwritten)
     long long ldy;
-    void *E;                     // intermediate, m1 rows of K*N elements, row stride lde
+    void *E;                     // intermediate, La rows of K*N elements, row stride lde
     long long lde;               // K*N rounded up to GF_EALIGN elements (jk_gf_lde)
     long long base[4];           // first unit: conversion, same-spin, E, gather
 };
@@ -86,5 +94,9 @@ int jk_gf_dense(long long nstr, const uint64_t *mask, const long long *off1, con
 // replacement lists (singles + diagonal pairs) of one spin
 int jk_gf_lists(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
                 int L, uint32_t *out, hipStream_t st);
+// alpha lists: out = in with bits 24..30 set to the E row of each entry; *missing (device)
+// counts entries whose pair is absent from the target's list (0 for a valid list)
+int jk_gf_rank(long long nstr, int L, const uint32_t *in, uint32_t *out, int *missing,
+               hipStream_t st);
 // (P|Q) padded to MP x MP in FP64 and FP32
 int jk_gf_m2pad(int m1, int MP, const double *m2sq, double *o64, float *o32, hipStream_t st);

@@ -108,6 +108,24 @@ __global__ void k_gf_lists(long long nstr, const uint64_t *__restrict__ mask,
     }
 }
 
+// E row of each alpha-list entry: the position of the entry's pair in its target's list
+__global__ void k_gf_rank(long long nstr, int L, const uint32_t *__restrict__ in,
+                          uint32_t *__restrict__ out, int *__restrict__ missing)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nstr * L) return;
+    const uint32_t w = in[idx];
+    const long long J = w & 0xFFFFu;
+    const uint32_t P = (w >> 16) & 0xFFu;
+    uint32_t pos = 0;
+    bool found = false;
+    if (J < nstr)
+        for (int e = 0; e < L; e++)
+            if (((in[J * L + e] >> 16) & 0xFFu) == P) { pos = (uint32_t)e; found = true; break; }
+    if (!found) atomicAdd(missing, 1);
+    out[idx] = (w & ~(0x7Fu << 24)) | (pos << 24);
+}
+
 __global__ void k_gf_m2pad(int m1, int MP, const double *__restrict__ sq, double *__restrict__ o64,
                            float *__restrict__ o32)
 {
@@ -296,13 +314,13 @@ template <> struct GeL<float> {
     }
 };
 
-// Products of one D tile: acc[i][j] += A(row tile w + 4i) . D(column tile j) over KS
-// reduction steps, straight-line code, the B vectors of the next group of steps read while
-// the products of the current group run.
-template <typename T, int BN, int KS, int NRT>
-__device__ __forceinline__ void gE_mma(typename Mf<T>::acc (&acc)[2][BN / 16],
-                                       const T (&a)[2][GF_MAXMP / 4],
-                                       const T (*Dt)[GeL<T>::LDT], int l)
+// Products of one D tile for the wave's row tile: acc[j] += A . D(column tile j) over the
+// KS reduction steps, straight-line code, the B vectors of the next group of steps read
+// while the products of the current group run (group gi = steps gi*VK .. gi*VK + VK - 1).
+template <typename T, int BN, int KS>
+__device__ __forceinline__ void gE_mma(typename Mf<T>::acc (&acc)[BN / 16],
+                                       const T (&a)[GF_MAXMP / 4], const T (*Dt)[GeL<T>::LDT],
+                                       int l)
 {
     constexpr int NCT = BN / 16, VK = GeL<T>::VK, NG = KS / VK;
     typedef typename GeL<T>::vec V;
@@ -321,48 +339,47 @@ __device__ __forceinline__ void gE_mma(typename Mf<T>::acc (&acc)[2][BN / 16],
 #pragma unroll
         for (int q = 0; q < VK; q++)
 #pragma unroll
-            for (int i = 0; i < NRT; i++)
-#pragma unroll
-                for (int j = 0; j < NCT; j++)
-                    acc[i][j] = Mf<T>::mma(a[i][gi * VK + q], GeL<T>::at(b[cb][j], q), acc[i][j]);
+            for (int j = 0; j < NCT; j++)
+                acc[j] = Mf<T>::mma(a[gi * VK + q], GeL<T>::at(b[cb][j], q), acc[j]);
     }
 }
 
 // the reduction depth KS = MP / 4 is a multiple of 4 up to GF_MAXMP / 4
-template <typename T, int BN, int NRT>
-__device__ __forceinline__ void gE_mma_ks(int KS, typename Mf<T>::acc (&acc)[2][BN / 16],
-                                          const T (&a)[2][GF_MAXMP / 4],
-                                          const T (*Dt)[GeL<T>::LDT], int l)
+template <typename T, int BN>
+__device__ __forceinline__ void gE_mma_ks(int KS, typename Mf<T>::acc (&acc)[BN / 16],
+                                          const T (&a)[GF_MAXMP / 4], const T (*Dt)[GeL<T>::LDT],
+                                          int l)
 {
     static_assert(GF_MAXMP == 128, "reduction depths below cover MP <= 128");
     switch (KS) {
-    case 4: gE_mma<T, BN, 4, NRT>(acc, a, Dt, l); break;
-    case 8: gE_mma<T, BN, 8, NRT>(acc, a, Dt, l); break;
-    case 12: gE_mma<T, BN, 12, NRT>(acc, a, Dt, l); break;
-    case 16: gE_mma<T, BN, 16, NRT>(acc, a, Dt, l); break;
-    case 20: gE_mma<T, BN, 20, NRT>(acc, a, Dt, l); break;
-    case 24: gE_mma<T, BN, 24, NRT>(acc, a, Dt, l); break;
-    case 28: gE_mma<T, BN, 28, NRT>(acc, a, Dt, l); break;
-    default: gE_mma<T, BN, 32, NRT>(acc, a, Dt, l); break;
+    case 4: gE_mma<T, BN, 4>(acc, a, Dt, l); break;
+    case 8: gE_mma<T, BN, 8>(acc, a, Dt, l); break;
+    case 12: gE_mma<T, BN, 12>(acc, a, Dt, l); break;
+    case 16: gE_mma<T, BN, 16>(acc, a, Dt, l); break;
+    case 20: gE_mma<T, BN, 20>(acc, a, Dt, l); break;
+    case 24: gE_mma<T, BN, 24>(acc, a, Dt, l); break;
+    case 28: gE_mma<T, BN, 28>(acc, a, Dt, l); break;
+    default: gE_mma<T, BN, 32>(acc, a, Dt, l); break;
     }
 }
 
-// E[P][c] = sum_Q (P|Q) D[Q][c] over columns c = (k, Ja, Ib): the tile of D is built in LDS
-// from the beta lists (each (Q, column) receives at most one entry) and multiplied by the
-// (P|Q) fragments that every wave keeps in registers (wave w: row tiles w and w + 4),
-// reloaded when the task changes.  Each work-group takes a contiguous run of column tiles,
-// so consecutive tiles share the vector row X_k[Ja]: it is staged in LDS once (rows of up to
-// GF_XROW strings) and the beta-list gathers read it there.  Columns past the end of the
-// last tile repeat the last column (their products are not stored), so every lane of a part
-// handles the same number of list entries.
-#define GF_XROW 4096
+// E[p][c] = sum_Q (P_p|Q) D[Q][c] over columns c = (k, Ja, Ib), for the La rows p of the
+// alpha string Ja (P_p = pair of entry p of Ja's alpha list).  A column tile holds BN beta
+// strings of one row (k, Ja), so the tile shares Ja: wave w keeps the (P|Q) fragments of
+// rows 16w .. 16w+15 of Ja in registers (La <= 64: at most four row tiles, one per wave),
+// reloaded when the row changes, and the row X_k[Ja] is staged in LDS (rows of up to
+// GF_XROW_BYTES: 2048 strings FP64, 4096 FP32) for the beta-list gathers.  The tile of D is built in LDS from the beta
+// lists (each (Q, column) receives at most one entry); columns past the end of the row
+// repeat its last column (their products are not stored).
+#define GF_XROW_BYTES 16384
 template <typename T, int BN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gE(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gE(
     const GfHeader *__restrict__ H, const GfTask *__restrict__ tasks)
 {
     constexpr int LDT = GeL<T>::LDT;
     __shared__ __align__(16) T Dt[BN][LDT];
-    __shared__ T xs[GF_XROW];
+    constexpr int XROW = GF_XROW_BYTES / (int)sizeof(T);
+    __shared__ T xs[XROW];
     constexpr int NCT = BN / 16, PARTS = 256 / BN, KSMAX = GF_MAXMP / 4;
     constexpr int EMAX = (GF_MAXLIST + PARTS - 1) / PARTS;   // list entries per thread
     const int tid = threadIdx.x, l = tid & 63;
@@ -376,15 +393,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     // task of the current tile and the first unit past it
     int ti = find_task(tasks, n, u0, GF_E);
     long long tend = ti + 1 < n ? tasks[ti + 1].base[GF_E] : total;
+    // tile u of a task: row r = (k, Ja) = k * Na + Ja, beta strings Ib0 .. Ib0 + ncol - 1
+    auto geom = [](const GfTask &t, long long u, long long &r, int &Ib0, int &ncol) {
+        const long long tpr = (t.g.nb + BN - 1) / BN, q = u - t.base[GF_E];
+        r = q / tpr;
+        Ib0 = (int)(q - r * tpr) * BN;
+        ncol = (int)std::min<long long>(BN, t.g.nb - Ib0);
+    };
     // beta-list entries of this thread's column for a tile (issued one tile ahead, so their
     // global loads overlap the products of the current tile); entry i is e = part + i*PARTS
     uint32_t lcur[EMAX], lnxt[EMAX];
     auto fetch = [&](long long u, uint32_t (&dst)[EMAX]) {
         if (u >= u1) return;
         const GfTask &t = tasks[u < tend ? ti : find_task(tasks, n, u, GF_E)];
-        const long long N = t.g.na * t.g.nb, ncols = N * t.K;
-        const long long c = std::min((u - t.base[GF_E]) * BN + col, ncols - 1);
-        const long long Ib = (c % N) % t.g.nb;
+        long long r;
+        int Ib0, ncol;
+        geom(t, u, r, Ib0, ncol);
+        const long long Ib = Ib0 + std::min(col, ncol - 1);
         const auto lst = gptr(t.g.lb + Ib * t.g.Lb);
 #pragma unroll
         for (int i = 0; i < EMAX; i++) {
@@ -393,9 +418,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
     };
     fetch(u0, lcur);
-    T a[2][KSMAX];
-    int cur = -1;
-    long long staged = -1;        // (task, k, Ja) of the row in xs, as task * 2^40 + k * N + Ja*Nb
+    T a[KSMAX];
+    long long arow = -1;          // (task, row) of the fragments in a[] and of the row in xs
     for (long long u = u0; u < u1; u++) {
         if (u >= tend) {
             ti = find_task(tasks, n, u, GF_E);
@@ -403,47 +427,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
         const GfTask &t = tasks[ti];
         const GemmFci &g = t.g;
-        const int MP = g.MP, KS = MP / 4, RT = MP / 16, Lb = g.Lb;
-        const int nrt = (w < RT) + (w + 4 < RT);
-        if (ti != cur) {
-            // A fragment of step ks: row rt*16 + l%16, column (l/16)*KS + ks
+        const int MP = g.MP, KS = MP / 4, Lb = g.Lb, La = g.La;
+        const bool active = 16 * w < La;
+        long long r;
+        int Ib0, ncol;
+        geom(t, u, r, Ib0, ncol);
+        const long long Nb = g.nb, k = r / g.na, Ja = r - k * g.na;
+        const bool staged_row = Nb <= XROW;
+        const long long key = ((long long)ti << 40) + r;
+        if (key != arow) {
+            // A fragment of step ks for row p = 16w + l%16 of Ja: (P_p | Q) at column
+            // Q = (l/16)*KS + ks
+            const int p = 16 * w + l % 16;
+            const bool ok = active && p < La;
+            const int P = ok ? (int)((gptr(g.la)[Ja * La + p] >> 16) & 0xFFu) : 0;
             const T *M2 = mpad<T>(g);
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const int rt = w + 4 * i;
-#pragma unroll
-                for (int ks = 0; ks < KSMAX; ks++)
-                    a[i][ks] = (rt < RT && ks < KS)
-                                   ? M2[(long long)(rt * 16 + l % 16) * MP + (l / 16) * KS + ks]
-                                   : (T)0;
-            }
-            cur = ti;
-        }
-        const long long Nb = g.nb, N = g.na * g.nb, ncols = N * t.K;
-        const long long c0 = (u - t.base[GF_E]) * BN;
-        const long long clast = std::min(c0 + BN, ncols) - 1;
-        // one vector row for the whole tile: stage it
-        const bool one_row = Nb <= GF_XROW && (c0 / Nb) == (clast / Nb);
-        if (one_row) {
-            const long long key = ((long long)ti << 40) + (c0 / Nb) * Nb;
-            if (key != staged) {
-                const long long k = c0 / N, Ja = (c0 - k * N) / Nb;
+            for (int ks = 0; ks < KSMAX; ks++)
+                a[ks] = (ok && ks < KS) ? M2[(long long)P * MP + (l / 16) * KS + ks] : (T)0;
+            if (staged_row) {
                 const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
                 for (int i = tid; i < Nb; i += 256) xs[i] = xr[i];
-                staged = key;
             }
+            arow = key;
         }
         __syncthreads();   // row staged; the previous tile's entries are cleared
         {
             // all reads first (independent, one wait), then the scatter into the D tile
             T v[EMAX];
-            if (one_row) {
+            if (staged_row) {
 #pragma unroll
                 for (int i = 0; i < EMAX; i++)
                     if (part + i * PARTS < Lb) v[i] = xs[lcur[i] & 0xFFFFu];
             } else {
-                const long long c = std::min(c0 + col, clast);
-                const long long k = c / N, Ja = (c - k * N) / Nb;
                 const auto xr = gptr(xin<T>(t, (int)k) + Ja * Nb);
 #pragma unroll
                 for (int i = 0; i < EMAX; i++)
@@ -456,25 +472,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
         __syncthreads();   // D tile complete
         fetch(u + 1, lnxt);
-        typename Mf<T>::acc acc[2][NCT];
+        if (active) {
+            typename Mf<T>::acc acc[NCT];
 #pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < NCT; j++) acc[i][j] = Mf<T>::zero();
-        if (nrt == 2) gE_mma_ks<T, BN, 2>(KS, acc, a, Dt, l);
-        else if (nrt == 1) gE_mma_ks<T, BN, 1>(KS, acc, a, Dt, l);
-        const auto E = gptr_w((T *)t.E);
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-            const int rt = w + 4 * i;
-            if (rt >= RT) continue;
+            for (int j = 0; j < NCT; j++) acc[j] = Mf<T>::zero();
+            gE_mma_ks<T, BN>(KS, acc, a, Dt, l);
+            const auto E = gptr_w((T *)t.E);
+            const long long c0 = r * Nb + Ib0;
 #pragma unroll
             for (int j = 0; j < NCT; j++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int P = rt * 16 + Mf<T>::row(l, v);
-                    const long long cc = c0 + j * 16 + l % 16;
-                    if (P < g.m1 && cc < ncols) E[(long long)P * t.lde + cc] = acc[i][j][v];
+                    const int p = 16 * w + Mf<T>::row(l, v), cj = j * 16 + l % 16;
+                    if (p < La && cj < ncol) E[(long long)p * t.lde + c0 + cj] = acc[j][v];
                 }
         }
         __syncthreads();   // the products have read the D tile
@@ -487,7 +497,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
-// y_k[Ia][Ib] += sum over the alpha list of Ia of sign * E[P][k, Ja, Ib]; 256 elements per unit
+// y_k[Ia][Ib] += sum over the alpha list of Ia of sign * E[p][k, Ja, Ib], p the entry's E row
+// (bits 24..30); 256 elements per unit
 template <typename T>
 __global__ __launch_bounds__(256) void k_gga(const GfHeader *__restrict__ H,
                                              const GfTask *__restrict__ tasks)
@@ -512,14 +523,14 @@ __global__ __launch_bounds__(256) void k_gga(const GfHeader *__restrict__ H,
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const uint32_t wd = lst[e + q];
-                v[q] = Ek[(long long)((wd >> 16) & 0xFFu) * lde + (long long)(wd & 0xFFFFu) * Nb];
+                v[q] = Ek[(long long)((wd >> 24) & 0x7Fu) * lde + (long long)(wd & 0xFFFFu) * Nb];
             }
 #pragma unroll
             for (int q = 0; q < 8; q++) acc += (lst[e + q] >> 31) ? -v[q] : v[q];
         }
         for (; e < g.La; e++) {
             const uint32_t wd = lst[e];
-            const T v = Ek[(long long)((wd >> 16) & 0xFFu) * lde + (long long)(wd & 0xFFFFu) * Nb];
+            const T v = Ek[(long long)((wd >> 24) & 0x7Fu) * lde + (long long)(wd & 0xFFFFu) * Nb];
             acc += (wd >> 31) ? -v : v;
         }
         t.y[k * t.ldy + rem] += (double)acc;
@@ -550,6 +561,16 @@ int jk_gf_lists(long long nstr, const uint64_t *mask, const long long *off1, con
     return 0;
 }
 
+int jk_gf_rank(long long nstr, int L, const uint32_t *in, uint32_t *out, int *missing,
+               hipStream_t st)
+{
+    if (nstr * L == 0) return 0;
+    hipLaunchKernelGGL(k_gf_rank, dim3((unsigned)((nstr * L + 255) / 256)), dim3(256), 0, st, nstr, L,
+                       in, out, missing);
+    GF_CHECK();
+    return 0;
+}
+
 int jk_gf_m2pad(int m1, int MP, const double *m2sq, double *o64, float *o32, hipStream_t st)
 {
     hipLaunchKernelGGL(k_gf_m2pad, dim3((MP * MP + 255) / 256), dim3(256), 0, st, m1, MP, m2sq,
@@ -568,7 +589,7 @@ void jk_gf_units(int prec, int bn, const GfTask &t, long long units[4])
     const long long na = t.g.na, nb = t.g.nb, N = na * nb, cols = N * t.K;
     units[GF_CV] = prec == 64 ? 0 : (cols + 255) / 256;
     units[GF_SS] = (long long)t.K * ((na + 63) / 64) * ((nb + 63) / 64);
-    units[GF_E] = (cols + bn - 1) / bn;
+    units[GF_E] = (long long)t.K * na * ((nb + bn - 1) / bn);
     units[GF_GA] = (cols + 255) / 256;
 }
 
