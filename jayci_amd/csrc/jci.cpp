@@ -1186,7 +1186,8 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     auto cur_method = [&]() { return phase32 ? m32 : m64; };
     auto count_sigma = [&](int n) { if (phase32) w.ns32 += n; else w.ns64 += n; };
     int m = 0, it = 0, converged = 0;
-    std::vector<double> G, theta(JCI_MAXBASIS), Yf, rn(JCI_MAXROOTS, 0.0);
+    std::vector<double> G, Gk, Gblk, theta(JCI_MAXBASIS), Yf, rn(JCI_MAXROOTS, 0.0);
+    int gvalid = 0;
     double *dY;
     // initial space
     if (p->guess == JCI_GUESS_USER) {
@@ -1219,11 +1220,33 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     if (m < nroots) { jci_set_error("solve: initial space has fewer vectors than roots"); rc = -1; goto done; }
     if ((rc = sigma_block(w, 0, m, cur_method()))) goto done;
     count_sigma(m);
+    // the projected matrix V^T S is kept across iterations (leading dimension JCI_MAXBASIS):
+    // columns [0, gvalid) are current, so an iteration computes only V^T S_new for the
+    // vectors appended since; a collapse leaves diag(theta), a full sigma recompute 0
+    Gk.assign((size_t)JCI_MAXBASIS * JCI_MAXBASIS, 0.0);
     for (it = 1; it <= p->maxiter; it++) {
+        if (gvalid > m) gvalid = 0;
+        if (gvalid < m) {
+            const int nn = m - gvalid;
+            Gblk.assign((size_t)m * nn, 0.0);
+            if ((rc = gram_host(w, m, w.V, nn, w.S + (size_t)gvalid * w.ld, Gblk.data()))) goto done;
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < nn; j++) {
+                    const int J = gvalid + j;
+                    const double v = Gblk[(size_t)i * nn + j];
+                    if (i < gvalid) {
+                        Gk[(size_t)i * JCI_MAXBASIS + J] = Gk[(size_t)J * JCI_MAXBASIS + i] = v;
+                    } else if (i <= J) {
+                        // both new: symmetrise V_i.S_J and V_J.S_i
+                        const double u = 0.5 * (v + Gblk[(size_t)J * nn + (i - gvalid)]);
+                        Gk[(size_t)i * JCI_MAXBASIS + J] = Gk[(size_t)J * JCI_MAXBASIS + i] = u;
+                    }
+                }
+            gvalid = m;
+        }
         G.assign((size_t)m * m, 0.0);
-        if ((rc = gram_host(w, m, w.V, m, w.S, G.data()))) goto done;
         for (int i = 0; i < m; i++)
-            for (int j = 0; j < i; j++) G[i * m + j] = G[j * m + i] = 0.5 * (G[i * m + j] + G[j * m + i]);
+            for (int j = 0; j < m; j++) G[(size_t)i * m + j] = Gk[(size_t)i * JCI_MAXBASIS + j];
         Yf.assign((size_t)m * m, 0.0);
         jacobi_eig(m, G.data(), theta.data(), Yf.data());
         // residuals of the lowest ntrack Ritz pairs -> T (preconditioned), norms
@@ -1265,6 +1288,7 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             }
             if ((rc = sigma_block(w, 0, m, cur_method()))) goto done;
             count_sigma(m);
+            gvalid = 0;
             continue;
         }
         bool req_done = true;
@@ -1300,9 +1324,12 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             if ((rc = jk_rotate_inplace(m, kc, N, w.V, w.ld, dY, c->stream))) goto done;
             if ((rc = jk_rotate_inplace(m, kc, N, w.S, w.ld, dY, c->stream))) goto done;
             m = kc;
-            // the collapsed basis vectors are the Ritz vectors themselves
+            // the collapsed basis vectors are the Ritz vectors themselves: V^T S = diag(theta)
             Yf.assign((size_t)m * m, 0.0);
             for (int i = 0; i < m; i++) Yf[(size_t)i * m + i] = 1.0;
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++) Gk[(size_t)i * JCI_MAXBASIS + j] = i == j ? theta[i] : 0.0;
+            gvalid = m;
         }
         int kept = 0;
         if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;
