@@ -57,17 +57,38 @@ __global__ __launch_bounds__(256) void k_bgram(const BHeader *__restrict__ H,
     typedef double v4d __attribute__((ext_vector_type(4)));
     v4d acc[4];
     for (int t = 0; t < 4; t++) acc[t] = (v4d){0.0, 0.0, 0.0, 0.0};
-    for (long long base = r0; base < r1; base += BG_ROWS) {
-        int rows = (int)((r1 - base) < BG_ROWS ? (r1 - base) : BG_ROWS);
-        for (int idx = tid; idx < BG_ROWS * mt * 16; idx += 256) {
-            int c = idx / BG_ROWS, r = idx % BG_ROWS;
-            sA[r][c] = (c < m && r < rows) ? d.A[(long long)c * d.ld + base + r] : 0.0;
+    // the chunk's values this thread stages (element idx = tid + 256 q: column idx / BG_ROWS,
+    // row idx % BG_ROWS), loaded into registers one chunk ahead so that the loads of the
+    // next chunk are in flight during the products of this one
+    constexpr int QA = BG_ROWS * JCI_MAXBASIS / 256, QB = BG_ROWS * JCI_MAXBASIS / 256;
+    double ra[QA], rb[QB];
+    auto gload = [&](long long base) {
+        const int rows = (int)((r1 - base) < BG_ROWS ? (r1 - base) : BG_ROWS);
+#pragma unroll
+        for (int q = 0; q < QA; q++) {
+            const int idx = tid + 256 * q, c = idx / BG_ROWS, r = idx % BG_ROWS;
+            ra[q] = (idx < BG_ROWS * mt * 16 && c < m && r < rows) ? d.A[(long long)c * d.ld + base + r] : 0.0;
         }
-        for (int idx = tid; idx < BG_ROWS * wt * 16; idx += 256) {
-            int c = idx / BG_ROWS, r = idx % BG_ROWS;
-            sB[r][c] = (c < w && r < rows) ? d.B[(long long)c * d.ld + base + r] : 0.0;
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int idx = tid + 256 * q, c = idx / BG_ROWS, r = idx % BG_ROWS;
+            rb[q] = (idx < BG_ROWS * wt * 16 && c < w && r < rows) ? d.B[(long long)c * d.ld + base + r] : 0.0;
+        }
+    };
+    if (r0 < r1) gload(r0);
+    for (long long base = r0; base < r1; base += BG_ROWS) {
+#pragma unroll
+        for (int q = 0; q < QA; q++) {
+            const int idx = tid + 256 * q;
+            if (idx < BG_ROWS * mt * 16) sA[idx % BG_ROWS][idx / BG_ROWS] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int idx = tid + 256 * q;
+            if (idx < BG_ROWS * wt * 16) sB[idx % BG_ROWS][idx / BG_ROWS] = rb[q];
         }
         __syncthreads();
+        if (base + BG_ROWS < r1) gload(base + BG_ROWS);
         for (int t = 0; t < 4; t++) {
             int tile = wave + 4 * t;
             if (tile >= ntile) break;
@@ -180,9 +201,22 @@ __global__ __launch_bounds__(256) void k_bresidual(const BHeader *__restrict__ H
     for (long long row = r0 + threadIdx.x; row < r1; row += 256) {
         double res[JCI_MAXK];
         for (int r = 0; r < k; r++) res[r] = 0.0;
-        for (int i = 0; i < m; i++) {
-            double a = d.A[(long long)i * d.ld + row], s = d.B[(long long)i * d.ld + row];
-            for (int r = 0; r < k; r++) res[r] += sC[i * k + r] * (s - sT[r] * a);
+        // four basis vectors per step: their eight loads are issued together
+        int i = 0;
+        for (; i + 4 <= m; i += 4) {
+            double a[4], sv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                a[q] = d.A[(long long)(i + q) * d.ld + row];
+                sv[q] = d.B[(long long)(i + q) * d.ld + row];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                for (int r = 0; r < k; r++) res[r] += sC[(i + q) * k + r] * (sv[q] - sT[r] * a[q]);
+        }
+        for (; i < m; i++) {
+            double a = d.A[(long long)i * d.ld + row], sv = d.B[(long long)i * d.ld + row];
+            for (int r = 0; r < k; r++) res[r] += sC[i * k + r] * (sv - sT[r] * a);
         }
         double h = d.hd[row];
         for (int r = 0; r < k; r++) {
