@@ -597,18 +597,31 @@ __global__ __launch_bounds__(256) void k_gram(int m, int w, long long N,
     for (int t = 0; t < 4; t++) acc[t] = (v4d){0.0, 0.0, 0.0, 0.0};
     long long r0 = (long long)blockIdx.x * chunk;
     long long r1 = r0 + chunk < N ? r0 + chunk : N;
-    for (long long base = r0; base < r1; base += GRAM_ROWS) {
-        int rows = (int)((r1 - base) < GRAM_ROWS ? (r1 - base) : GRAM_ROWS);
-        // stage: element (vector c, row r) -> sV[r][c]; zero-pad to multiples of 16
-        for (int idx = tid; idx < GRAM_ROWS * mt * 16; idx += 256) {
-            int c = idx / GRAM_ROWS, r = idx % GRAM_ROWS;
-            sV[r][c] = (c < m && r < rows) ? V[(long long)c * ldv + base + r] : 0.0;
+    // stage: element (vector c, row r) -> sV[r][c], zero-padded to multiples of 16; element
+    // idx = tid + 256 q of the chunk is column idx / GRAM_ROWS, row idx % GRAM_ROWS.  The
+    // values are loaded into registers one chunk ahead, so the loads of the next chunk are
+    // in flight during the products of this one.
+    constexpr int Q = GRAM_ROWS * JCI_MAXBASIS / 256;
+    double rv[Q], rw[Q];
+    auto gload = [&](long long base) {
+        const int rows = (int)((r1 - base) < GRAM_ROWS ? (r1 - base) : GRAM_ROWS);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int idx = tid + 256 * q, c = idx / GRAM_ROWS, r = idx % GRAM_ROWS;
+            rv[q] = (idx < GRAM_ROWS * mt * 16 && c < m && r < rows) ? V[(long long)c * ldv + base + r] : 0.0;
+            rw[q] = (idx < GRAM_ROWS * wt * 16 && c < w && r < rows) ? W[(long long)c * ldw + base + r] : 0.0;
         }
-        for (int idx = tid; idx < GRAM_ROWS * wt * 16; idx += 256) {
-            int c = idx / GRAM_ROWS, r = idx % GRAM_ROWS;
-            sW[r][c] = (c < w && r < rows) ? W[(long long)c * ldw + base + r] : 0.0;
+    };
+    if (r0 < r1) gload(r0);
+    for (long long base = r0; base < r1; base += GRAM_ROWS) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int idx = tid + 256 * q;
+            if (idx < GRAM_ROWS * mt * 16) sV[idx % GRAM_ROWS][idx / GRAM_ROWS] = rv[q];
+            if (idx < GRAM_ROWS * wt * 16) sW[idx % GRAM_ROWS][idx / GRAM_ROWS] = rw[q];
         }
         __syncthreads();
+        if (base + GRAM_ROWS < r1) gload(base + GRAM_ROWS);
         for (int t = 0; t < 4; t++) {
             int tile = wave + 4 * t;
             if (tile >= ntile) break;
@@ -737,7 +750,20 @@ __global__ __launch_bounds__(256) void k_residual(int m, int k, long long N,
     for (long long n = r0 + threadIdx.x; n < r1; n += blockDim.x) {
         double res[JCI_MAXK];
         for (int r = 0; r < k; r++) res[r] = 0.0;
-        for (int i = 0; i < m; i++) {
+        // four basis vectors per step: their eight loads are issued together
+        int i = 0;
+        for (; i + 4 <= m; i += 4) {
+            double vi[4], si[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                vi[q] = V[(long long)(i + q) * ld + n];
+                si[q] = S[(long long)(i + q) * ld + n];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                for (int r = 0; r < k; r++) res[r] += sY[(i + q) * k + r] * (si[q] - sT[r] * vi[q]);
+        }
+        for (; i < m; i++) {
             double vi = V[(long long)i * ld + n], si = S[(long long)i * ld + n];
             for (int r = 0; r < k; r++) res[r] += sY[i * k + r] * (si - sT[r] * vi);
         }
