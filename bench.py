@@ -180,16 +180,17 @@ PEAK_MFMA = {64: 78.6, 32: 157.3}   # MI355X dense MFMA peaks, TFLOP/s
 
 def roofline_dense(J, ctx, spec, seed):
     """Dense-string sigma (k_gE + k_gss + k_gga, one vector) on one full-CI instance, FP64
-    and FP32: algorithmic FLOPs 2*La*m1*N (opposite-spin product: the La = a(n-a) + a pairs
-    an alpha string connects through, against all m1 pairs of (P|Q)) + 2*(Na^2*Nb + Na*Nb^2)
-    (same-spin products) over the HIP-event time of one sigma."""
+    and FP32: algorithmic FLOPs 2*La*m1*N (opposite-spin product: the La = a(n-a) + 1 rows
+    an alpha string connects through, its a(n-a) single-replacement pairs and one row for
+    its occupied diagonal pairs together, against all m1 pairs of (P|Q))
+    + 2*(Na^2*Nb + Na*Nb^2) (same-spin products) over the HIP-event time of one sigma."""
     from math import comb
     sp, h = make_instance(J, ctx, spec, seed, J.DIRECT_TABLES)
     n = spec["norbs"]
     na, nb = comb(n, spec["na"]), comb(n, spec["nb"])
     N = na * nb
     m1 = n * (n + 1) // 2
-    La = spec["na"] * (n - spec["na"]) + spec["na"]
+    La = spec["na"] * (n - spec["na"]) + 1
     flops = 2.0 * La * m1 * N + 2.0 * (na * na * nb + na * nb * nb)
     out = {}
     traffic = {}

@@ -269,8 +269,8 @@ struct jci_ham {
     GemmFci gf;
     double *d_gfH64[2];
     float *d_gfH32[2];
-    double *d_gfM64;
-    float *d_gfM32;
+    double *d_gfM64, *d_gfD64;
+    float *d_gfM32, *d_gfD32;
     uint32_t *d_gfla, *d_gflb;
     void *d_gfE;                // opposite-spin intermediate E
     size_t gfE_bytes;
@@ -346,7 +346,8 @@ static int build_gf(jci_ham *h)
     if ((double)(na * na + (same ? 0 : nb * nb)) * 12.0 > 8e9) return 0;
     const int sa = s->na * (n - s->na), sb = s->nb * (n - s->nb);
     if (h->nexc1[0] != na * sa || h->nexc1[1] != nb * sb) return 0;
-    const int La = sa + s->na, Lb = sb + s->nb;
+    // alpha lists: the singles and one entry for the occupied diagonal pairs together
+    const int La = sa + (s->na > 0 ? 1 : 0), Lb = sb + s->nb;
     if (La > GF_MAXLIST || Lb > GF_MAXLIST) return 0;
     JCI_CHECK(hipMalloc(&h->d_gfH64[0], sizeof(double) * na * na));
     JCI_CHECK(hipMalloc(&h->d_gfH32[0], sizeof(float) * na * na));
@@ -366,9 +367,12 @@ static int build_gf(jci_ham *h)
     JCI_CHECK(hipMalloc(&h->d_gfM64, sizeof(double) * MP * MP));
     JCI_CHECK(hipMalloc(&h->d_gfM32, sizeof(float) * MP * MP));
     RCHECK(jk_gf_m2pad(m1, MP, h->d_m2sq, h->d_gfM64, h->d_gfM32, c->stream));
+    JCI_CHECK(hipMalloc(&h->d_gfD64, sizeof(double) * na * MP));
+    JCI_CHECK(hipMalloc(&h->d_gfD32, sizeof(float) * na * MP));
+    RCHECK(jk_gf_diagrows(na, MP, s->d_mask[0], h->d_gfM64, h->d_gfD64, h->d_gfD32, c->stream));
     JCI_CHECK(hipMalloc(&h->d_gfla, sizeof(uint32_t) * na * La));
     JCI_CHECK(hipMalloc(&h->d_gflb, sizeof(uint32_t) * nb * Lb));
-    RCHECK(jk_gf_lists(nb, s->d_mask[1], h->d_off1[1], h->d_e1[1], Lb, h->d_gflb, c->stream));
+    RCHECK(jk_gf_lists(nb, s->d_mask[1], h->d_off1[1], h->d_e1[1], Lb, 0, h->d_gflb, c->stream));
     {
         // alpha lists with the E row of each entry (the pair's position in the target's list)
         uint32_t *raw = nullptr;
@@ -376,7 +380,7 @@ static int build_gf(jci_ham *h)
         JCI_CHECK(hipMalloc(&raw, sizeof(uint32_t) * na * La));
         JCI_CHECK(hipMalloc(&d_missing, sizeof(int)));
         JCI_CHECK(hipMemsetAsync(d_missing, 0, sizeof(int), c->stream));
-        int rc = jk_gf_lists(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], La, raw, c->stream);
+        int rc = jk_gf_lists(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], La, 1, raw, c->stream);
         if (!rc) rc = jk_gf_rank(na, La, raw, h->d_gfla, d_missing, c->stream);
         if (!rc && hipMemcpyAsync(&missing, d_missing, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
             rc = -2;
@@ -405,6 +409,8 @@ static int build_gf(jci_ham *h)
     g.M32 = h->d_gfM32;
     g.la = h->d_gfla;
     g.lb = h->d_gflb;
+    g.D64 = h->d_gfD64;
+    g.D32 = h->d_gfD32;
     h->gf_bn64 = GF_BN64_DEFAULT;
     h->gf_bn32 = GF_BN32_DEFAULT;
     h->have_gf = 1;
@@ -559,6 +565,7 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     if (h->d_gfH64[1] != h->d_gfH64[0]) { hipFree(h->d_gfH64[1]); hipFree(h->d_gfH32[1]); }
     hipFree(h->d_gfH64[0]); hipFree(h->d_gfH32[0]);
     hipFree(h->d_gfM64); hipFree(h->d_gfM32); hipFree(h->d_gfla); hipFree(h->d_gflb);
+    hipFree(h->d_gfD64); hipFree(h->d_gfD32);
     hipFree(h->d_gfE); hipFree(h->d_gfX32); hipFree(h->d_gfT);
     hipFree(h->d_gft); hipFree(h->d_gfh);
     if (h->h_gft) hipHostFree(h->h_gft);

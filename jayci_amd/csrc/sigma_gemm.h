@@ -24,6 +24,13 @@
 // alpha list, and each alpha-list entry carries, in bits 24..30, the position of its pair
 // in the list of its target string.
 //
+// The diagonal pairs of an alpha string all land on the string itself with sign +, so
+// their E rows are only ever summed: the alpha list holds one entry for them, pair
+// GF_PDIAG, whose row of (P|Q) is sum_{k occupied in Ja} (kk|Q).  La = a(n-a) + 1 instead
+// of a(n-a) + a (46 rows instead of 50 for 5 electrons in 14 orbitals: three row tiles of
+// 16 instead of four).  Beta lists keep one entry per diagonal pair, since D is needed
+// for every Q.
+//
 // Matrix elements are those of the Slater-Condon rules in slater.h (the corrected
 // same-spin doubles of mpi_source/action_util.c:215-240).
 #pragma once
@@ -35,6 +42,7 @@
 
 // list entry: bits 0..15 target string, 16..23 orbital pair, bit 31 sign; alpha lists also
 // bits 24..30: position of the pair in the target string's list (the E row)
+#define GF_PDIAG 0xFF         // alpha-list pair: sum of the string's occupied diagonal pairs
 struct GemmFci {
     long long na, nb;            // alpha / beta strings
     int m1, MP;                  // orbital pairs, padded to a multiple of 16
@@ -44,6 +52,8 @@ struct GemmFci {
     const double *M64;           // (P|Q), MP x MP, zero padded
     const float *M32;
     const uint32_t *la, *lb;     // [Na][La], [Nb][Lb]
+    const double *D64;           // [Na][MP]: the (P|Q) row of each alpha string's GF_PDIAG
+    const float *D32;            // entry, sum of (kk|Q) over its occupied orbitals k
 };
 
 // One instance's sigma in a grouped launch.  Every kernel of the sigma walks a flattened
@@ -91,12 +101,16 @@ int jk_gf_group(int prec, int bn, const GfHeader *d_hdr, const GfTask *d_tasks,
 int jk_gf_dense(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
                 const double *v1, const long long *off2, const int2 *e2, const double *v2,
                 const double *m1, const double *m2, double *H, hipStream_t st);
-// replacement lists (singles + diagonal pairs) of one spin
+// replacement lists of one spin: the singles, then one entry per occupied diagonal pair
+// (diag_one = 0, beta) or one GF_PDIAG entry for all of them (diag_one = 1, alpha)
 int jk_gf_lists(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
-                int L, uint32_t *out, hipStream_t st);
+                int L, int diag_one, uint32_t *out, hipStream_t st);
 // alpha lists: out = in with bits 24..30 set to the E row of each entry; *missing (device)
 // counts entries whose pair is absent from the target's list (0 for a valid list)
 int jk_gf_rank(long long nstr, int L, const uint32_t *in, uint32_t *out, int *missing,
                hipStream_t st);
 // (P|Q) padded to MP x MP in FP64 and FP32
 int jk_gf_m2pad(int m1, int MP, const double *m2sq, double *o64, float *o32, hipStream_t st);
+// rows of the GF_PDIAG entries: o[Ja][Q] = sum_{k occupied in Ja} M64[kk][Q], FP64 and FP32
+int jk_gf_diagrows(long long nstr, int MP, const uint64_t *mask, const double *M64, double *o64,
+                   float *o32, hipStream_t st);

@@ -90,7 +90,7 @@ __global__ void k_gf_dense(long long nstr, const uint64_t *__restrict__ mask,
 
 __global__ void k_gf_lists(long long nstr, const uint64_t *__restrict__ mask,
                            const long long *__restrict__ off1, const int2 *__restrict__ e1, int L,
-                           uint32_t *__restrict__ out)
+                           int diag_one, uint32_t *__restrict__ out)
 {
     long long I = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nstr) return;
@@ -101,6 +101,10 @@ __global__ void k_gf_lists(long long nstr, const uint64_t *__restrict__ mask,
         int i = en.y & 63, f = (en.y >> 6) & 63;
         uint32_t P = (uint32_t)jci_tri(i, f);
         o[n++] = (uint32_t)en.x | (P << 16) | ((uint32_t)((en.y >> 12) & 1) << 31);
+    }
+    if (diag_one) {
+        if (mask[I] && n < L) o[n++] = (uint32_t)I | ((uint32_t)GF_PDIAG << 16);
+        return;
     }
     for (uint64_t x = mask[I]; x && n < L; x &= x - 1) {
         int k = jci_ctz(x);
@@ -124,6 +128,23 @@ __global__ void k_gf_rank(long long nstr, int L, const uint32_t *__restrict__ in
             if (((in[J * L + e] >> 16) & 0xFFu) == P) { pos = (uint32_t)e; found = true; break; }
     if (!found) atomicAdd(missing, 1);
     out[idx] = (w & ~(0x7Fu << 24)) | (pos << 24);
+}
+
+__global__ void k_gf_diagrows(long long nstr, int MP, const uint64_t *__restrict__ mask,
+                              const double *__restrict__ M64, double *__restrict__ o64,
+                              float *__restrict__ o32)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nstr * MP) return;
+    const long long J = idx / MP;
+    const int Q = (int)(idx - J * MP);
+    double s = 0.0;
+    for (uint64_t x = mask[J]; x; x &= x - 1) {
+        const int k = jci_ctz(x);
+        s += M64[jci_tri(k, k) * MP + Q];
+    }
+    o64[idx] = s;
+    o32[idx] = (float)s;
 }
 
 __global__ void k_gf_m2pad(int m1, int MP, const double *__restrict__ sq, double *__restrict__ o64,
@@ -151,6 +172,9 @@ template <> __device__ __forceinline__ const float *hmat<float>(const GemmFci &g
 template <typename T> __device__ __forceinline__ const T *mpad(const GemmFci &g);
 template <> __device__ __forceinline__ const double *mpad<double>(const GemmFci &g) { return g.M64; }
 template <> __device__ __forceinline__ const float *mpad<float>(const GemmFci &g) { return g.M32; }
+template <typename T> __device__ __forceinline__ const T *drows(const GemmFci &g);
+template <> __device__ __forceinline__ const double *drows<double>(const GemmFci &g) { return g.D64; }
+template <> __device__ __forceinline__ const float *drows<float>(const GemmFci &g) { return g.D32; }
 
 // input vector k of a task in the arithmetic type
 template <typename T> __device__ __forceinline__ const T *xin(const GfTask &t, int k);
@@ -250,9 +274,11 @@ __device__ __forceinline__ void mma_tile(typename Mf<T>::acc (&acc)[2][2], const
     }
 }
 
-// same-spin parts: y_k = Haa X_k + X_k Hbb on 64 x 64 tiles (k, alpha rows, beta columns)
+// same-spin parts: y_k = Haa X_k + X_k Hbb on 64 x 64 tiles (k, alpha rows, beta columns).
+// Four blocks per CU (at most 128 registers, 33 KB of LDS in FP64): the 1024 tiles of a
+// 2002 x 2002 string space are then resident at once instead of running as 768 + 256.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gss(const GfHeader *__restrict__ H,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gss(const GfHeader *__restrict__ H,
                                              const GfTask *__restrict__ tasks)
 {
     __shared__ T As[2][16][65];
@@ -437,14 +463,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const long long key = ((long long)ti << 40) + r;
         if (key != arow) {
             // A fragment of step ks for row p = 16w + l%16 of Ja: (P_p | Q) at column
-            // Q = (l/16)*KS + ks
+            // Q = (l/16)*KS + ks; for the GF_PDIAG entry, Ja's row of sums of (kk | Q)
             const int p = 16 * w + l % 16;
             const bool ok = active && p < La;
             const int P = ok ? (int)((gptr(g.la)[Ja * La + p] >> 16) & 0xFFu) : 0;
-            const T *M2 = mpad<T>(g);
+            const T *Arow = P == GF_PDIAG ? drows<T>(g) + Ja * MP : mpad<T>(g) + (long long)P * MP;
 #pragma unroll
             for (int ks = 0; ks < KSMAX; ks++)
-                a[ks] = (ok && ks < KS) ? M2[(long long)P * MP + (l / 16) * KS + ks] : (T)0;
+                a[ks] = (ok && ks < KS) ? Arow[(l / 16) * KS + ks] : (T)0;
             if (staged_row) {
                 const T *__restrict__ xr = xin<T>(t, (int)k) + Ja * Nb;
                 for (int i = tid; i < Nb; i += 256) xs[i] = xr[i];
@@ -553,10 +579,10 @@ int jk_gf_dense(long long nstr, const uint64_t *mask, const long long *off1, con
 }
 
 int jk_gf_lists(long long nstr, const uint64_t *mask, const long long *off1, const int2 *e1,
-                int L, uint32_t *out, hipStream_t st)
+                int L, int diag_one, uint32_t *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_gf_lists, dim3((unsigned)((nstr + 127) / 128)), dim3(128), 0, st, nstr,
-                       mask, off1, e1, L, out);
+                       mask, off1, e1, L, diag_one, out);
     GF_CHECK();
     return 0;
 }
@@ -579,6 +605,16 @@ int jk_gf_m2pad(int m1, int MP, const double *m2sq, double *o64, float *o32, hip
     return 0;
 }
 
+int jk_gf_diagrows(long long nstr, int MP, const uint64_t *mask, const double *M64, double *o64,
+                   float *o32, hipStream_t st)
+{
+    if (nstr * MP == 0) return 0;
+    hipLaunchKernelGGL(k_gf_diagrows, dim3((unsigned)((nstr * MP + 255) / 256)), dim3(256), 0, st,
+                       nstr, MP, mask, M64, o64, o32);
+    GF_CHECK();
+    return 0;
+}
+
 int jk_gf_bn_valid(int prec, int bn)
 {
     return prec == 64 ? (bn == 16 || bn == 32) : (bn == 32 || bn == 64);
@@ -593,16 +629,33 @@ void jk_gf_units(int prec, int bn, const GfTask &t, long long units[4])
     units[GF_GA] = (cols + 255) / 256;
 }
 
-// persistent grid caps per kernel
+// persistent grid caps of the grid-stride kernels (conversion, gather)
 static const long long gf_cap[4] = {4096, 2048, 2048, 4096};
+
+// blocks of 256 threads of a kernel resident on the whole device at once
+static long long resident_blocks(const void *kernel)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) != hipSuccess)
+        return 0;
+    return (long long)cus * per;
+}
 
 template <typename T, int BN>
 static int gf_group_t(const GfHeader *d_hdr, const GfTask *d_tasks, const GfHeader *hh,
                       hipStream_t st)
 {
+    // The E kernel splits its units into equal consecutive ranges, one per block, and the
+    // same-spin kernel walks whole 64 x 64 tiles: both run one wave of resident blocks
+    // (a grid past residency leaves a last, partly filled wave of blocks)
+    static const long long cap_e = std::max(256LL, resident_blocks((const void *)k_gE<T, BN>));
+    static const long long cap_ss = std::max(256LL, resident_blocks((const void *)k_gss<T>));
+    const long long cap[4] = {gf_cap[GF_CV], cap_ss, cap_e, gf_cap[GF_GA]};
     unsigned g[4];
     for (int i = 0; i < 4; i++)
-        g[i] = (unsigned)(hh ? std::max<long long>(1, std::min(hh->total[i], gf_cap[i])) : gf_cap[i]);
+        g[i] = (unsigned)(hh ? std::max<long long>(1, std::min(hh->total[i], cap[i])) : cap[i]);
     if (sizeof(T) == 4) {
         hipLaunchKernelGGL(k_gcv, dim3(g[GF_CV]), dim3(256), 0, st, d_hdr, d_tasks);
         GF_CHECK();
