@@ -177,6 +177,11 @@ typedef struct jci_ham jci_ham;
 #define JCI_GUESS_UNIT 4    /* unit vectors on determinants 0..krymin-1 (the MPI driver's
                                prediagr 2, mpi_source/pdavidson.c:418-431) */
 
+#define JCI_RESTART_RITZ 0  /* the lowest krymin Ritz vectors (source/davidson.c:91-288) */
+#define JCI_RESTART_GDK 1   /* those and the previous iteration's Ritz vectors of the tracked
+                               pairs (GD+k: the restart keeps the direction each pair last
+                               moved in) */
+
 #define JCI_MAXROOTS 8
 
 const char *jci_last_error(void);
@@ -233,6 +238,9 @@ typedef struct {
                              |r|^2 < etol * gap, gap = distance of its Ritz value to the
                              nearest Ritz value outside the requested set (|r|^2 / gap bounds
                              the eigenvalue error) */
+    double buftol;        /* buffer pairs stop expanding once their residual norm is below
+                             buftol (0: they expand until restol, like the requested roots) */
+    int restart;          /* JCI_RESTART_*: what a full subspace collapses onto */
 } jci_solve_params;
 
 typedef struct {

@@ -21,6 +21,7 @@ PAIRS_FP64, PAIRS_FP32, PAIRS_UPPER, DIRECT_TABLES = 1, 2, 4, 8
  SIGMA_FAST64, SIGMA_FAST32, SIGMA_FCI64, SIGMA_FCI32, SIGMA_GEMM64, SIGMA_GEMM32) = range(12)
 NMETHODS = 12
 PREC_FP64, PREC_MIXED = 0, 1
+RESTART_RITZ, RESTART_GDK = 0, 1
 GUESS_BLOCK, GUESS_USER, GUESS_DIAG, GUESS_UNIT = 1, 2, 3, 4
 MAXROOTS = 8
 SYNTH_ENUC = 10.0
@@ -37,7 +38,7 @@ class SolveParams(ctypes.Structure):
     _fields_ = [("maxiter", _i), ("krymin", _i), ("krymax", _i), ("nroots", _i), ("refdim", _i),
                 ("restol", _d), ("precision", _i), ("mixed_switch", _d), ("sigma_fp64", _i),
                 ("sigma_fp32", _i), ("guess", _i), ("totfrze", _d), ("verbose", _i),
-                ("nbuffer", _i), ("etol", _d)]
+                ("nbuffer", _i), ("etol", _d), ("buftol", _d), ("restart", _i)]
 
 
 class SolveStats(ctypes.Structure):
@@ -365,32 +366,19 @@ class Hamiltonian:
 
     def params(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
                precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
-               sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0, etol=0.0):
+               sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0, etol=0.0,
+               buftol=0.0, restart=RESTART_RITZ):
         return make_params(self.space.ndets, nroots, restol, maxiter, krymin, krymax, refdim,
                            precision, mixed_switch, guess, totfrze, sigma_fp64, sigma_fp32, verbose,
-                           nbuffer, etol)
+                           nbuffer, etol, buftol, restart)
 
     def solve(self, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
               precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, guess_vectors=None,
               totfrze=0.0, sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, vectors=True,
-              verbose=False, check=True, nbuffer=0, etol=0.0):
-        p = SolveParams()
-        _L.jci_solve_defaults(ctypes.byref(p))
-        p.nroots = nroots
-        p.restol = restol
-        p.maxiter = maxiter
-        p.krymin = krymin if krymin is not None else max(nroots + 1, min(4, self.space.ndets))
-        p.krymax = krymax if krymax is not None else min(max(p.krymin + 2 * nroots, 16), 64)
-        p.refdim = refdim
-        p.precision = precision
-        p.mixed_switch = mixed_switch
-        p.guess = guess
-        p.totfrze = totfrze
-        p.sigma_fp64 = sigma_fp64
-        p.sigma_fp32 = sigma_fp32
-        p.verbose = 1 if verbose else 0
-        p.nbuffer = nbuffer
-        p.etol = etol
+              verbose=False, check=True, nbuffer=0, etol=0.0, buftol=0.0, restart=RESTART_RITZ):
+        p = make_params(self.space.ndets, nroots, restol, maxiter, krymin, krymax, refdim,
+                        precision, mixed_switch, guess, totfrze, sigma_fp64, sigma_fp32, verbose,
+                        nbuffer, etol, buftol, restart)
         N = self.space.ndets
         ev = np.zeros(nroots)
         vec = np.zeros((nroots, N)) if vectors else None
@@ -471,7 +459,7 @@ def read_moints(path, norbs, type1n=1):
 def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=None, refdim=64,
                 precision=PREC_FP64, mixed_switch=1e-4, guess=GUESS_BLOCK, totfrze=0.0,
                 sigma_fp64=SIGMA_AUTO, sigma_fp32=SIGMA_AUTO, verbose=False, nbuffer=0,
-                etol=0.0):
+                etol=0.0, buftol=0.0, restart=RESTART_RITZ):
     p = SolveParams()
     _L.jci_solve_defaults(ctypes.byref(p))
     p.nroots = nroots
@@ -489,6 +477,8 @@ def make_params(ndets, nroots=1, restol=1e-6, maxiter=200, krymin=None, krymax=N
     p.verbose = 1 if verbose else 0
     p.nbuffer = nbuffer
     p.etol = etol
+    p.buftol = buftol
+    p.restart = restart
     return p
 
 

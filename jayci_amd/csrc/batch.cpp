@@ -55,6 +55,9 @@ struct BInst {
     long long ns64 = 0, ns32 = 0;
     std::vector<double> theta, Yf;
     std::vector<double> G;             // projected matrix V^T S (m x m), kept across steps
+    std::vector<double> Yp;            // previous step's Ritz vectors of the tracked pairs (GD+k
+    int mp = 0, np = 0;                // restart): np vectors of mp rows, zero below mp
+    std::vector<double> Cg;            // this step's GD+k restart block (m x kc, row-major)
     int gnew = 0;                      // first basis column whose sigma arrived this step
     double rn[JCI_MAXROOTS] = {0};
     double ev[JCI_MAXROOTS] = {0};
@@ -797,6 +800,7 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
                 ok = rall[r] * rall[r] < s.p.etol * gap;
             }
             if (!ok && r < s.nroots) req_done = false;
+            if (r >= s.nroots && rall[r] < s.p.buftol) ok = true;
             if (!ok) s.todo.push_back(r);
         }
         if (req_done) { s.done = 1; s.t_done = tnow(); continue; }
@@ -810,7 +814,26 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             if (kc < s.nroots) kc = s.nroots;
             if (kc + (int)s.todo.size() > s.krymax) s.todo.resize(s.krymax - kc);
             s.kc = kc;
+            s.Cg.clear();
+            if (s.p.restart == JCI_RESTART_GDK && s.np > 0) {
+                // GD+k: the lowest kc Ritz vectors and the previous step's ones
+                const int kmax = s.krymax - (int)s.todo.size();
+                std::vector<double> Gc((size_t)kmax * kmax);
+                s.Cg.resize((size_t)s.m * kmax);
+                s.kc = jci_gdk_restart(s.m, s.G.data(), s.Yf.data(), kc, s.mp, s.np, s.Yp.data(),
+                                       kmax, s.Cg.data(), Gc.data());
+                s.Cg.resize((size_t)s.m * s.kc);
+                Gc.resize((size_t)s.kc * s.kc);
+                s.Yp.swap(Gc);   // holds C^T G C until the rotation is issued
+            }
+            // this step's Ritz vectors of the tracked pairs: after the collapse they are
+            // the leading columns of the basis
+            s.np = std::min(nt, kc);
             collapse.push_back(act[t]);
+        } else {
+            s.mp = s.m;
+            s.np = nt;
+            s.Yp.assign(s.Yf.begin(), s.Yf.begin() + (size_t)nt * s.m);
         }
         grow.push_back(act[t]);
     }
@@ -826,8 +849,11 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
         std::vector<size_t> cofs;
         size_t co = 0;
         auto add_rot = [&](BInst &s, int i, int kc, bool with_s) {
-            for (int r = 0; r < s.m; r++)
-                for (int j = 0; j < kc; j++) b->hcoef[co + (size_t)r * kc + j] = s.Yf[(size_t)j * s.m + r];
+            if (!s.Cg.empty() && with_s)
+                memcpy(b->hcoef + co, s.Cg.data(), sizeof(double) * (size_t)s.m * kc);
+            else
+                for (int r = 0; r < s.m; r++)
+                    for (int j = 0; j < kc; j++) b->hcoef[co + (size_t)r * kc + j] = s.Yf[(size_t)j * s.m + r];
             ids.push_back(i); A.push_back(s.V); O.push_back(s.V); m.push_back(s.m); k.push_back(kc);
             cofs.push_back(co);
             if (with_s) {
@@ -851,13 +877,22 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             s.G.clear();
             s.sig_first = 0;
             s.sig_count = s.m;
+            s.np = 0;
         }
         for (int i : collapse) {
             BInst &s = b->in[i];
             const int kc = s.kc;
-            s.G.assign((size_t)kc * kc, 0.0);
-            for (int j = 0; j < kc; j++) s.G[(size_t)j * kc + j] = s.theta[j];
+            if (!s.Cg.empty()) {
+                s.G.swap(s.Yp);   // C^T G C
+                s.Cg.clear();
+            } else {
+                s.G.assign((size_t)kc * kc, 0.0);
+                for (int j = 0; j < kc; j++) s.G[(size_t)j * kc + j] = s.theta[j];
+            }
             s.m = kc;
+            s.mp = kc;
+            s.Yp.assign((size_t)s.np * s.mp, 0.0);
+            for (int j = 0; j < s.np; j++) s.Yp[(size_t)j * s.mp + j] = 1.0;
         }
     }
     if (grow.empty()) { b->phase_seconds[3] += tnow() - tp; return 0; }

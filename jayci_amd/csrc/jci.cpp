@@ -992,6 +992,52 @@ void jci_jacobi_eig(int n, const double *a, double *eval, double *vec)
     jacobi_eig(n, a, eval, vec);
 }
 
+int jci_gdk_restart(int m, const double *G, const double *Yf, int kc, int mp, int np,
+                    const double *Yp, int kmax, double *C, double *Gc)
+{
+    std::vector<double> cols;   // column-major, m rows
+    int n = 0;
+    kc = std::min(kc, kmax);
+    for (int j = 0; j < kc; j++, n++) cols.insert(cols.end(), Yf + (size_t)j * m, Yf + (size_t)j * m + m);
+    std::vector<double> v(m);
+    for (int j = 0; j < np && n < kmax; j++) {
+        for (int i = 0; i < m; i++) v[i] = i < mp ? Yp[(size_t)j * mp + i] : 0.0;
+        for (int pass = 0; pass < 2; pass++)
+            for (int q = 0; q < n; q++) {
+                const double *c = &cols[(size_t)q * m];
+                double d = 0.0;
+                for (int i = 0; i < m; i++) d += c[i] * v[i];
+                for (int i = 0; i < m; i++) v[i] -= d * c[i];
+            }
+        double nr = 0.0;
+        for (int i = 0; i < m; i++) nr += v[i] * v[i];
+        nr = sqrt(nr);
+        if (nr < 1e-8) continue;
+        for (int i = 0; i < m; i++) v[i] /= nr;
+        cols.insert(cols.end(), v.begin(), v.end());
+        n++;
+    }
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) C[(size_t)i * n + j] = cols[(size_t)j * m + i];
+    std::vector<double> GC((size_t)m * n);   // G C, row-major
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) {
+            double t = 0.0;
+            for (int l = 0; l < m; l++) t += G[(size_t)i * m + l] * cols[(size_t)j * m + l];
+            GC[(size_t)i * n + j] = t;
+        }
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b <= a; b++) {
+            double ab = 0.0, ba = 0.0;
+            for (int i = 0; i < m; i++) {
+                ab += cols[(size_t)a * m + i] * GC[(size_t)i * n + b];
+                ba += cols[(size_t)b * m + i] * GC[(size_t)i * n + a];
+            }
+            Gc[(size_t)a * n + b] = Gc[(size_t)b * n + a] = 0.5 * (ab + ba);
+        }
+    return n;
+}
+
 JCI_API int jci_jacobi(int n, const double *a, double *eval, double *vec)
 {
     if (n < 1) return -1;
@@ -1195,6 +1241,10 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     int m = 0, it = 0, converged = 0;
     std::vector<double> G, Gk, Gblk, theta(JCI_MAXBASIS), Yf, rn(JCI_MAXROOTS, 0.0);
     int gvalid = 0;
+    // Ritz vectors of the tracked pairs at the previous iteration (GD+k restart), np vectors
+    // of mp rows in the current basis (zero below mp)
+    std::vector<double> Yp;
+    int mp = 0, np = 0;
     double *dY;
     // initial space
     if (p->guess == JCI_GUESS_USER) {
@@ -1296,6 +1346,7 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             if ((rc = sigma_block(w, 0, m, cur_method()))) goto done;
             count_sigma(m);
             gvalid = 0;
+            np = 0;
             continue;
         }
         bool req_done = true;
@@ -1308,6 +1359,7 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
                 ok = rall[r] * rall[r] < p->etol * gap;
             }
             if (!ok && r < nroots) req_done = false;
+            if (r >= nroots && rall[r] < p->buftol) ok = true;
             if (!ok) todo.push_back(r);
         }
         if (req_done) { converged = 1; break; }
@@ -1324,19 +1376,41 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             kc = std::min(kc, krymax - k);
             if (kc < nroots) kc = nroots;
             if (kc + k > krymax) k = krymax - kc;   // drop trailing buffer corrections
-            std::vector<double> Yc((size_t)m * kc);
-            for (int i = 0; i < m; i++)
-                for (int j = 0; j < kc; j++) Yc[(size_t)i * kc + j] = Yf[(size_t)j * m + i];
-            if ((rc = upload_small(w, Yc.data(), m * kc, &dY))) goto done;
-            if ((rc = jk_rotate_inplace(m, kc, N, w.V, w.ld, dY, c->stream))) goto done;
-            if ((rc = jk_rotate_inplace(m, kc, N, w.S, w.ld, dY, c->stream))) goto done;
-            m = kc;
-            // the collapsed basis vectors are the Ritz vectors themselves: V^T S = diag(theta)
+            if (p->restart == JCI_RESTART_GDK && np > 0) {
+                // GD+k: also keep the previous iteration's Ritz vectors
+                std::vector<double> C((size_t)m * (krymax - k)), Gc((size_t)(krymax - k) * (krymax - k));
+                const int nc = jci_gdk_restart(m, G.data(), Yf.data(), kc, mp, np, Yp.data(),
+                                               krymax - k, C.data(), Gc.data());
+                if ((rc = upload_small(w, C.data(), m * nc, &dY))) goto done;
+                if ((rc = jk_rotate_inplace(m, nc, N, w.V, w.ld, dY, c->stream))) goto done;
+                if ((rc = jk_rotate_inplace(m, nc, N, w.S, w.ld, dY, c->stream))) goto done;
+                m = nc;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) Gk[(size_t)i * JCI_MAXBASIS + j] = Gc[(size_t)i * m + j];
+            } else {
+                std::vector<double> Yc((size_t)m * kc);
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < kc; j++) Yc[(size_t)i * kc + j] = Yf[(size_t)j * m + i];
+                if ((rc = upload_small(w, Yc.data(), m * kc, &dY))) goto done;
+                if ((rc = jk_rotate_inplace(m, kc, N, w.V, w.ld, dY, c->stream))) goto done;
+                if ((rc = jk_rotate_inplace(m, kc, N, w.S, w.ld, dY, c->stream))) goto done;
+                m = kc;
+                // the collapsed basis vectors are the Ritz vectors themselves: V^T S = diag(theta)
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) Gk[(size_t)i * JCI_MAXBASIS + j] = i == j ? theta[i] : 0.0;
+            }
             Yf.assign((size_t)m * m, 0.0);
             for (int i = 0; i < m; i++) Yf[(size_t)i * m + i] = 1.0;
-            for (int i = 0; i < m; i++)
-                for (int j = 0; j < m; j++) Gk[(size_t)i * JCI_MAXBASIS + j] = i == j ? theta[i] : 0.0;
             gvalid = m;
+            // this iteration's Ritz vectors are the leading kc columns of the new basis
+            mp = m;
+            np = std::min(nt, kc);
+            Yp.assign((size_t)np * mp, 0.0);
+            for (int j = 0; j < np; j++) Yp[(size_t)j * mp + j] = 1.0;
+        } else {
+            mp = m;
+            np = nt;
+            Yp.assign(Yf.begin(), Yf.begin() + (size_t)nt * m);
         }
         int kept = 0;
         if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;

@@ -82,6 +82,14 @@ struct SolveOp {
 int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, double *evals,
                    double *evecs, jci_solve_stats *stats, int nout);
 long long jci_ham_ndets(jci_ham *h);
+// GD+k restart block: the lowest kc Ritz vectors of the projected matrix G (row-major m x m;
+// eigenvectors Yf, vector j at Yf + j*m) followed by the np previous Ritz vectors Yp (vector j
+// at Yp + j*mp, mp <= m rows, zero below), each made orthonormal to the columns before it by
+// two Gram-Schmidt passes (one left with norm below 1e-8 is dropped), at most kmax columns.
+// C (m x kmax) receives the columns row-major (element (i, j) at C[i*cols + j]) and Gc
+// (kmax x kmax) C^T G C row-major (cols x cols).  Returns cols.
+int jci_gdk_restart(int m, const double *G, const double *Yf, int kc, int mp, int np,
+                    const double *Yp, int kmax, double *C, double *Gc);
 // jci_solve returning the lowest max(nroots, nout) Ritz vectors
 int jci_solve_nout(jci_ham *h, const jci_solve_params *p, const double *guess, double *evals,
                    double *evecs, jci_solve_stats *stats, int nout);

@@ -16,8 +16,8 @@ import time
 
 import numpy as np
 
-from . import (Batch, Hamiltonian, Space, DIRECT_TABLES, GUESS_DIAG, PREC_FP64, SYNTH_ENUC,
-               synth_integrals)
+from . import (Batch, Hamiltonian, Space, DIRECT_TABLES, GUESS_DIAG, PREC_FP64, RESTART_GDK,
+               SYNTH_ENUC, synth_integrals)
 
 
 def instance_spec(rng):
@@ -45,16 +45,19 @@ def ndets_of(spec):
     return None
 
 
-def solve_params(spec, N, restol=1e-6, nbuffer=1, etol=1e-9, kfac=8):
+def solve_params(spec, N, restol=1e-6, nbuffer=1, etol=1e-9, kfac=8, buftol=1e-3,
+                 restart=RESTART_GDK):
     """Davidson settings of a request (the same for the served and the one-at-a-time path):
     block of nroots + nbuffer tracked pairs, subspace up to kfac times that (at least 12);
     residual below restol and |r|^2 / gap below etol (eigenvalue error well under 1e-8 Ha
-    also next to a close unrequested state)."""
+    also next to a close unrequested state).  The buffer pairs stop expanding once their
+    residual is below buftol, and a full subspace restarts GD+k style (the lowest Ritz
+    vectors and the previous iteration's ones): both cut the sigma applications per solve."""
     nr = spec["nroots"]
     nt = nr + nbuffer
     return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N),
                 krymax=min(max(kfac * nt, 12), N, 64), refdim=min(64, N), guess=GUESS_DIAG,
-                totfrze=SYNTH_ENUC, nbuffer=nbuffer, etol=etol)
+                totfrze=SYNTH_ENUC, nbuffer=nbuffer, etol=etol, buftol=buftol, restart=restart)
 
 
 class Request:

@@ -52,6 +52,37 @@ def test_batch_matches_dense_and_alone(J, orc, ctx, precision, graphs):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["PREC_FP64", "PREC_MIXED"])
+def test_batch_gdk_restart_and_buffer_tolerance(J, orc, ctx, precision):
+    """Batched solves with GD+k restarts and buffer pairs stopping at residual 1e-3 (small
+    subspaces, so every instance restarts several times) against dense eigenvalues and the
+    standalone solver with the same settings."""
+    prec = getattr(J, precision)
+    items = [_build(J, orc, ctx, spec, 300 + i) for i, spec in enumerate(INSTANCES)]
+    b = J.Batch(ctx, 16, 16, graphs=True)
+    kws, slots = [], []
+    for sp, h, w, nr in items:
+        kw = dict(nroots=nr, restol=1e-7, krymin=min(nr + 2, sp.ndets),
+                  krymax=min(2 * (nr + 1) + 2, sp.ndets), refdim=min(48, sp.ndets),
+                  precision=prec, guess=J.GUESS_DIAG, nbuffer=1, buftol=1e-3,
+                  restart=J.RESTART_GDK)
+        kws.append(kw)
+        slots.append(b.add(h, h.params(**kw)))
+    b.run(600)
+    for (sp, h, w, nr), s, kw in zip(items, slots, kws):
+        ev, vec, st = b.result(s, vectors=True)
+        assert st["done"] == 1, st
+        alone, _, _ = h.solve(**kw)
+        np.testing.assert_allclose(ev, alone, rtol=0, atol=1e-8)
+        if w is not None:
+            np.testing.assert_allclose(ev, w[:nr], rtol=0, atol=1e-8)
+        hv = h.sigma(vec, method=J.SIGMA_FAST64)
+        for r in range(nr):
+            assert np.linalg.norm(hv[r] - ev[r] * vec[r]) < 2e-7
+        b.release(s)
+    b.close()
+
+
 def test_batch_continuous_admission(J, orc, ctx):
     """Instances added while others are mid-solve give the same result as alone."""
     items = [_build(J, orc, ctx, spec, 200 + i) for i, spec in enumerate(INSTANCES)]

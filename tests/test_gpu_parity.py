@@ -185,6 +185,27 @@ def test_davidson_precision_and_methods(J, orc, ctx, precision, flags, sig64, si
         assert st["nsigma32"] > 0 and st["nsigma64"] > 0
 
 
+@pytest.mark.parametrize("precision", ["PREC_FP64", "PREC_MIXED"])
+@pytest.mark.parametrize("spec", [(3, 3, 8, 0, 8, 0, 1), (3, 3, 8, 2, 2, 2, 1)])
+def test_davidson_gdk_restart_and_buffer_tolerance(J, orc, ctx, precision, spec):
+    """GD+k restarts (the previous iteration's Ritz vectors kept next to the lowest ones) and
+    buffer pairs that stop expanding at residual 1e-3: a subspace of 10 for 3 roots + 1
+    buffer collapses every two iterations, and the roots still match dense eigenvalues to
+    1e-8 with residuals below the tolerance."""
+    gs, os_, m1, m2 = _spaces(J, orc, ctx, spec, seed=8)
+    H = os_.dense_h(m1, m2)
+    w = np.linalg.eigh(H)[0]
+    h = J.Hamiltonian(gs, m1, m2, J.DIRECT_TABLES)
+    ev, vec, st = h.solve(nroots=3, restol=1e-7, krymin=5, krymax=10, refdim=40,
+                          precision=getattr(J, precision), guess=J.GUESS_DIAG, nbuffer=1,
+                          buftol=1e-3, restart=J.RESTART_GDK)
+    np.testing.assert_allclose(ev, w[:3], rtol=0, atol=1e-8)
+    for r in range(3):
+        assert np.linalg.norm(H @ vec[r] - ev[r] * vec[r]) < 2e-7
+        for q in range(r):
+            assert abs(vec[r] @ vec[q]) < 1e-10
+
+
 def test_user_guess_and_maxiter(J, orc, ctx):
     gs, os_, m1, m2 = _spaces(J, orc, ctx, (3, 3, 8, 0, 8, 0, 1), seed=4)
     h = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64)
@@ -224,10 +245,11 @@ def test_buffer_root_finds_lowest_state(J, ctx):
     sp = J.Space(ctx, 5, 5, 10)
     h = J.Hamiltonian(sp, m1, m2, J.DIRECT_TABLES)
     for prec in (J.PREC_MIXED, J.PREC_FP64):
-        ev, _, st = h.solve(nroots=1, restol=1e-6, krymin=3, krymax=12, refdim=64,
-                            guess=J.GUESS_DIAG, totfrze=J.SYNTH_ENUC, vectors=False,
-                            precision=prec, nbuffer=1)
-        assert abs(ev[0] - 36.10636502) < 1e-7, (prec, ev, st)
+        for restart, buftol in ((J.RESTART_RITZ, 0.0), (J.RESTART_GDK, 1e-3)):
+            ev, _, st = h.solve(nroots=1, restol=1e-6, krymin=3, krymax=12, refdim=64,
+                                guess=J.GUESS_DIAG, totfrze=J.SYNTH_ENUC, vectors=False,
+                                precision=prec, nbuffer=1, buftol=buftol, restart=restart)
+            assert abs(ev[0] - 36.10636502) < 1e-7, (prec, restart, ev, st)
 
 
 def test_mrcisd_16e60o_sampled_rows_bitexact(J, orc, ctx):
