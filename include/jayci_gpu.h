@@ -215,6 +215,10 @@ int jci_ham_create(jci_space *sp, const double *moints1, const double *moints2, 
 void jci_ham_destroy(jci_ham *h);
 long long jci_ham_npairs(const jci_ham *h);
 int jci_ham_hdiag(jci_ham *h, double *out);
+/* indices of the nb lowest diagonal elements, ascending by (value, index): the determinants
+ * of the JCI_GUESS_DIAG initial block (no counterpart in the reference, whose guess takes
+ * the leading refdim determinants, source/initguess_sbd.c:24) */
+int jci_ham_lowest_diag(jci_ham *h, int nb, long long *idx);
 /* connected off-diagonal entries of the given rows (both triangles, unsorted):
  * row t's entries at [t*cap, t*cap + counts[t]) */
 int jci_ham_rows(jci_ham *h, const long long *rows, int nrows, long long cap, long long *cols,

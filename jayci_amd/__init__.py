@@ -75,6 +75,7 @@ _sig("jci_ham_create", _i, [_P, _P, _P, _i, ctypes.POINTER(_P)])
 _sig("jci_ham_destroy", None, [_P])
 _sig("jci_ham_npairs", _ll, [_P])
 _sig("jci_ham_hdiag", _i, [_P, _P])
+_sig("jci_ham_lowest_diag", _i, [_P, _i, _P])
 _sig("jci_ham_rows", _i, [_P, _P, _i, _ll, _P, _P, _P, _P, _P])
 _sig("jci_sigma_dev", _i, [_P, _i, _i, _P, _P])
 _sig("jci_sigma", _i, [_P, _i, _i, _P, _P])
@@ -299,6 +300,12 @@ class Hamiltonian:
     def hdiag(self):
         out = np.zeros(self.space.ndets)
         _check(_L.jci_ham_hdiag(self._h, _p(out)), "jci_ham_hdiag")
+        return out
+
+    def lowest_diag(self, nb):
+        """Indices of the nb lowest diagonal elements, ascending by (value, index)."""
+        out = np.zeros(nb, dtype=np.int64)
+        _check(_L.jci_ham_lowest_diag(self._h, nb, _p(out)), "jci_ham_lowest_diag")
         return out
 
     def rows(self, rows, cap=None):

@@ -47,6 +47,8 @@ struct BInst {
     long long N = 0, ld = 0;
     double *V = nullptr, *S = nullptr, *T = nullptr, *X = nullptr, *Y = nullptr;
     float *X32 = nullptr;
+    void *mem = nullptr;               // the one device block V .. X32 are carved from
+    size_t mem_bytes = 0;
     int m = 0, krymin = 0, krymax = 0, nroots = 0, ntrack = 0;
     int phase32 = 0;
     int sig_first = 0, sig_count = 0;  // basis columns whose sigma is pending
@@ -113,6 +115,9 @@ struct jci_batch {
     int bn64, bn32;    // E tile widths of the dense-string sigma
     long long steps;
     double sigma_seconds;
+    // device blocks of released instances, kept for reuse (oldest first)
+    std::vector<std::pair<void *, size_t>> cache;
+    size_t cache_bytes;
     // host wall time per step phase: 0 sigma (all of run_sigma), 1 Gram + Rayleigh-Ritz,
     // 2 residuals and decisions, 3 selection / collapse / hand-over, 4 Gram-Schmidt and
     // orthonormalisation, 5 append
@@ -196,9 +201,60 @@ JCI_API int jci_batch_create(jci_ctx *c, int capacity, int max_norbs, jci_batch 
     return 0;
 }
 
-static void free_inst(BInst &s)
+// Instance memory: one device block per instance, and released blocks are kept (up to
+// JB_CACHE_BYTES, oldest freed first) for the next instance that fits in one at most twice
+// its size.  A served stream admits and releases instances of a few GB each (V and S of a
+// 4M-determinant request with a 32-vector subspace are 2 GB); fresh hipMalloc / hipFree
+// of those took 15 and 1.3 ms per request, time in which no step runs.
+#define JB_CACHE_BYTES (16ULL << 30)
+static void cache_flush(jci_batch *b)
 {
-    hipFree(s.V); hipFree(s.S); hipFree(s.T); hipFree(s.X); hipFree(s.Y); hipFree(s.X32);
+    for (auto &e : b->cache) hipFree(e.first);
+    b->cache.clear();
+    b->cache_bytes = 0;
+}
+
+static void *block_get(jci_batch *b, size_t bytes, size_t *got)
+{
+    int best = -1;
+    for (size_t i = 0; i < b->cache.size(); i++) {
+        const size_t sz = b->cache[i].second;
+        if (sz >= bytes && sz / 2 <= bytes && (best < 0 || sz < b->cache[best].second)) best = (int)i;
+    }
+    if (best >= 0) {
+        void *p = b->cache[best].first;
+        *got = b->cache[best].second;
+        b->cache_bytes -= *got;
+        b->cache.erase(b->cache.begin() + best);
+        return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        cache_flush(b);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    *got = bytes;
+    return p;
+}
+
+// the caller has synchronised the stream: no kernel still uses the block
+static void free_inst(jci_batch *b, BInst &s)
+{
+    if (s.mem) {
+        b->cache.push_back({s.mem, s.mem_bytes});
+        b->cache_bytes += s.mem_bytes;
+        while (b->cache_bytes > JB_CACHE_BYTES) {
+            hipFree(b->cache.front().first);
+            b->cache_bytes -= b->cache.front().second;
+            b->cache.erase(b->cache.begin());
+        }
+    }
+    s.mem = nullptr;
+    s.mem_bytes = 0;
     s.V = s.S = s.T = s.X = s.Y = nullptr;
     s.X32 = nullptr;
 }
@@ -208,7 +264,8 @@ JCI_API void jci_batch_destroy(jci_batch *b)
     if (!b) return;
     hipSetDevice(b->c->device);
     hipStreamSynchronize(b->c->stream);
-    for (auto &s : b->in) if (s.used) free_inst(s);
+    for (auto &s : b->in) if (s.used) free_inst(b, s);
+    cache_flush(b);
     for (auto &g : b->graphs) hipGraphExecDestroy(g.exec);
     hipFree(b->dH); hipHostFree(b->hH);
     hipFree(b->dD);
@@ -292,19 +349,30 @@ JCI_API int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, i
         return -1;
     }
     size_t vb = sizeof(double) * (size_t)s.ld;
-    if (hipMalloc(&s.V, vb * s.krymax) != hipSuccess || hipMalloc(&s.S, vb * s.krymax) != hipSuccess ||
-        hipMalloc(&s.T, vb * JCI_MAXK) != hipSuccess ||
-        (!s.gf && (hipMalloc(&s.X, vb * JCI_MAXK) != hipSuccess ||
-                   hipMalloc(&s.Y, vb * JCI_MAXK) != hipSuccess)) ||
-        hipMalloc(&s.X32, sizeof(float) * s.ld * JCI_MAXK) != hipSuccess) {
-        free_inst(s);
+    // V, S: krymax vectors; T: JCI_MAXK; X, Y (table sigma only): JCI_MAXK; X32: JCI_MAXK FP32
+    // (every piece a multiple of 512 bytes, as ld is a multiple of 64)
+    const size_t need = vb * (2 * (size_t)s.krymax + (s.gf ? 1 : 3) * (size_t)JCI_MAXK) +
+                        sizeof(float) * (size_t)s.ld * JCI_MAXK;
+    s.mem = block_get(b, need, &s.mem_bytes);
+    if (s.mem) {
+        char *q = (char *)s.mem;
+        s.V = (double *)q; q += vb * s.krymax;
+        s.S = (double *)q; q += vb * s.krymax;
+        s.T = (double *)q; q += vb * JCI_MAXK;
+        if (!s.gf) {
+            s.X = (double *)q; q += vb * JCI_MAXK;
+            s.Y = (double *)q; q += vb * JCI_MAXK;
+        }
+        s.X32 = (float *)q;
+    }
+    if (!s.mem) {
         jci_set_error("batch: out of device memory for an instance of %lld determinants", s.N);
         return -3;
     }
     s.used = 1;
     s.t_add = tnow();
     int rc = jci_ham_guess_block(h, p->refdim, s.krymin, p->guess == JCI_GUESS_DIAG, s.V, s.ld);
-    if (rc) { free_inst(s); s.used = 0; return rc; }
+    if (rc) { free_inst(b, s); s.used = 0; return rc; }
     s.m = s.krymin;
     s.sig_first = 0;
     s.sig_count = s.krymin;
@@ -317,7 +385,7 @@ JCI_API int jci_batch_release(jci_batch *b, int slot)
 {
     if (slot < 0 || slot >= b->cap || !b->in[slot].used) { jci_set_error("batch: bad slot"); return -1; }
     hipStreamSynchronize(b->c->stream);
-    free_inst(b->in[slot]);
+    free_inst(b, b->in[slot]);
     b->in[slot] = BInst();
     return 0;
 }

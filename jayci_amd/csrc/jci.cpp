@@ -1155,6 +1155,53 @@ static int orthonormalize_new(Work &w, int m, int k, int *kept)
     return 0;
 }
 
+// indices of the nb lowest diagonal elements, ascending by (value, index): each chunk's nb
+// lowest selected on the device (jk_lowest_chunks), the candidates merged here; past
+// JCI_GUESS_CHUNK the whole diagonal is sorted here
+static int lowest_diag(jci_ctx *c, const double *d_hd, long long N, int nb, long long *idx)
+{
+    if (nb < 1 || nb > N) { jci_set_error("lowest_diag: %d of %lld", nb, N); return -1; }
+    std::vector<double> hv;
+    std::vector<long long> hi;
+    if (nb <= JCI_GUESS_CHUNK) {
+        const size_t ncand = (size_t)((N + JCI_GUESS_CHUNK - 1) / JCI_GUESS_CHUNK) * nb;
+        void *cand = nullptr;
+        JCI_CHECK(hipMalloc(&cand, (sizeof(double) + sizeof(long long)) * ncand));
+        double *d_cv = (double *)cand;
+        long long *d_ci = (long long *)(d_cv + ncand);
+        hv.resize(ncand);
+        hi.resize(ncand);
+        int rc = jk_lowest_chunks(N, d_hd, nb, d_cv, d_ci, c->stream);
+        if (!rc && (hipMemcpyAsync(hv.data(), d_cv, sizeof(double) * ncand, hipMemcpyDeviceToHost,
+                                   c->stream) != hipSuccess ||
+                    hipMemcpyAsync(hi.data(), d_ci, sizeof(long long) * ncand, hipMemcpyDeviceToHost,
+                                   c->stream) != hipSuccess ||
+                    hipStreamSynchronize(c->stream) != hipSuccess))
+            rc = -2;
+        hipFree(cand);
+        if (rc) return rc;
+    } else {
+        hv.resize(N);
+        hi.resize(N);
+        JCI_CHECK(hipMemcpyAsync(hv.data(), d_hd, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+        for (long long i = 0; i < N; i++) hi[i] = i;
+    }
+    std::vector<size_t> ord(hv.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = i;
+    std::partial_sort(ord.begin(), ord.begin() + nb, ord.end(), [&](size_t a, size_t b) {
+        return hv[a] < hv[b] || (hv[a] == hv[b] && hi[a] < hi[b]);
+    });
+    for (int i = 0; i < nb; i++) idx[i] = hi[ord[i]];
+    return 0;
+}
+
+JCI_API int jci_ham_lowest_diag(jci_ham *h, int nb, long long *idx)
+{
+    JCI_CHECK(hipSetDevice(h->sp->ctx->device));
+    return lowest_diag(h->sp->ctx, h->d_hd, h->sp->h.ndets, nb, idx);
+}
+
 static int guess_block(Work &w, const jci_solve_params *p, int nguess)
 {
     SolveOp *op = w.op;
@@ -1163,25 +1210,18 @@ static int guess_block(Work &w, const jci_solve_params *p, int nguess)
     if (nb < nguess) nb = (int)std::min<long long>(nguess, N);
     std::vector<long long> idx(nb);
     if (p->guess == JCI_GUESS_DIAG) {
-        std::vector<double> hd(N);
-        JCI_CHECK(hipMemcpyAsync(hd.data(), op->d_hd, sizeof(double) * N, hipMemcpyDeviceToHost,
-                                 w.c->stream));
-        JCI_CHECK(hipStreamSynchronize(w.c->stream));
-        std::vector<long long> ord(N);
-        for (long long i = 0; i < N; i++) ord[i] = i;
-        std::partial_sort(ord.begin(), ord.begin() + nb, ord.end(), [&](long long a, long long b) {
-            return hd[a] < hd[b] || (hd[a] == hd[b] && a < b);
-        });
-        for (int i = 0; i < nb; i++) idx[i] = ord[i];
+        RCHECK(lowest_diag(w.c, op->d_hd, N, nb, idx.data()));
     } else {
         for (int i = 0; i < nb; i++) idx[i] = i;
     }
-    long long *d_idx;
-    uint64_t *d_m;
-    double *d_H;
-    JCI_CHECK(hipMalloc(&d_idx, sizeof(long long) * nb));
-    JCI_CHECK(hipMalloc(&d_m, sizeof(uint64_t) * 2 * nb));
-    JCI_CHECK(hipMalloc(&d_H, sizeof(double) * nb * nb));
+    // device scratch in one block: indices, masks, the block of H and its eigenvectors
+    void *scratch = nullptr;
+    JCI_CHECK(hipMalloc(&scratch, sizeof(long long) * nb + sizeof(uint64_t) * 2 * nb +
+                                      sizeof(double) * ((size_t)nb * nb + (size_t)nb * nguess)));
+    long long *d_idx = (long long *)scratch;
+    uint64_t *d_m = (uint64_t *)(d_idx + nb);
+    double *d_H = (double *)(d_m + 2 * nb);
+    double *d_U = d_H + (size_t)nb * nb;
     JCI_CHECK(hipMemcpyAsync(d_idx, idx.data(), sizeof(long long) * nb, hipMemcpyHostToDevice,
                              w.c->stream));
     RCHECK(op->masks(op->self, nb, d_idx, d_m, d_m + nb));
@@ -1192,14 +1232,12 @@ static int guess_block(Work &w, const jci_solve_params *p, int nguess)
     JCI_CHECK(hipStreamSynchronize(w.c->stream));
     jacobi_eig(nb, H.data(), ev.data(), U.data());
     // first nguess eigenvectors of the block, scattered to their determinants
-    double *d_U;
-    JCI_CHECK(hipMalloc(&d_U, sizeof(double) * nb * nguess));
     JCI_CHECK(hipMemcpyAsync(d_U, U.data(), sizeof(double) * nb * nguess, hipMemcpyHostToDevice,
                              w.c->stream));
     JCI_CHECK(hipMemsetAsync(w.V, 0, sizeof(double) * w.ld * nguess, w.c->stream));
     RCHECK(jk_scatter_guess(nb, nguess, d_idx, d_U, w.V, w.ld, w.c->stream));
     JCI_CHECK(hipStreamSynchronize(w.c->stream));
-    hipFree(d_idx); hipFree(d_m); hipFree(d_H); hipFree(d_U);
+    hipFree(scratch);
     return 0;
 }
 

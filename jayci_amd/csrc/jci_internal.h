@@ -136,6 +136,12 @@ int jk_colnorms(int k, long long N, const double *X, long long ld, double *parti
 int jk_scale_cols(int k, long long N, double *X, long long ld, const double *s, hipStream_t st);
 int jk_dense_block(int nb, const uint64_t *ma_rows, const uint64_t *mb_rows, int norbs,
                    const double *m1, const double *m2, double *H, hipStream_t st);
+// the nb (<= JCI_GUESS_CHUNK) lowest diagonal elements of each chunk of JCI_GUESS_CHUNK
+// determinants: values cv and indices ci, chunk c at [c*nb, c*nb + nb), ascending (value,
+// index); entries past N come as (+inf, LLONG_MAX)
+#define JCI_GUESS_CHUNK 2048
+int jk_lowest_chunks(long long N, const double *hd, int nb, double *cv, long long *ci,
+                     hipStream_t st);
 
 // drop-in path on explicit determinant lists
 int jk_list_hdiag(long long n, const uint64_t *ma, const uint64_t *mb, int norbs,

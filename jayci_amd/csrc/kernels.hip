@@ -1099,6 +1099,58 @@ __global__ void k_scatter_guess(int nb, int nvec, const long long *__restrict__ 
     V[(long long)j * ld + idx[r]] = U[(long long)j * nb + r];
 }
 
+// The nb lowest diagonal elements of each chunk of JCI_GUESS_CHUNK determinants, ascending
+// by value with ties by index (the order of the host selection), one block per chunk:
+// bitonic sort of the chunk in LDS (entries past N sort last as +inf).  The host merges the
+// chunks' candidates.
+__global__ __launch_bounds__(256) void k_lowest_chunks(long long N, const double *__restrict__ hd,
+                                                       int nb, double *__restrict__ cv,
+                                                       long long *__restrict__ ci)
+{
+    __shared__ double sv[JCI_GUESS_CHUNK];
+    __shared__ long long si[JCI_GUESS_CHUNK];
+    const long long base = (long long)blockIdx.x * JCI_GUESS_CHUNK;
+    for (int t = threadIdx.x; t < JCI_GUESS_CHUNK; t += 256) {
+        const long long g = base + t;
+        sv[t] = g < N ? hd[g] : __builtin_huge_val();
+        si[t] = g < N ? g : 0x7FFFFFFFFFFFFFFFLL;
+    }
+    __syncthreads();
+    for (int k = 2; k <= JCI_GUESS_CHUNK; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < JCI_GUESS_CHUNK; t += 256) {
+                const int q = t ^ j;
+                if (q > t) {
+                    const bool up = (t & k) == 0;
+                    const bool gt = sv[t] > sv[q] || (sv[t] == sv[q] && si[t] > si[q]);
+                    if (gt == up) {
+                        const double v = sv[t];
+                        sv[t] = sv[q];
+                        sv[q] = v;
+                        const long long i = si[t];
+                        si[t] = si[q];
+                        si[q] = i;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int t = threadIdx.x; t < nb; t += 256) {
+        cv[(long long)blockIdx.x * nb + t] = sv[t];
+        ci[(long long)blockIdx.x * nb + t] = si[t];
+    }
+}
+
+int jk_lowest_chunks(long long N, const double *hd, int nb, double *cv, long long *ci,
+                     hipStream_t st)
+{
+    if (nb < 1 || nb > JCI_GUESS_CHUNK || N < 1) return -1;
+    const long long nch = (N + JCI_GUESS_CHUNK - 1) / JCI_GUESS_CHUNK;
+    hipLaunchKernelGGL(k_lowest_chunks, dim3((unsigned)nch), dim3(256), 0, st, N, hd, nb, cv, ci);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 int jk_scatter_guess(int nb, int nvec, const long long *idx, const double *Yb, double *V,
                      long long ld, hipStream_t st)
 {

@@ -206,6 +206,31 @@ def test_davidson_gdk_restart_and_buffer_tolerance(J, orc, ctx, precision, spec)
             assert abs(vec[r] @ vec[q]) < 1e-10
 
 
+@pytest.mark.parametrize("n,a,nb", [(10, 4, 64), (8, 3, 200), (6, 2, 225), (12, 4, 64)])
+def test_lowest_diag_selection(J, ctx, n, a, nb):
+    """The chunked device selection of the guess determinants equals a stable sort of the
+    diagonal (value, then index): one chunk and several (44,100 and 245,025 determinants
+    span 22 and 120 chunks of 2048), nb up to the whole space (6 orbitals, 2 electrons:
+    225 determinants), and a diagonal with ties (the integer-valued diagonal below)."""
+    m1, m2 = J.synth_integrals(n, 31)
+    sp = J.Space(ctx, a, a, n)
+    h = J.Hamiltonian(sp, m1, m2, J.DIRECT_TABLES)
+    hd = h.hdiag()
+    want = np.argsort(hd, kind="stable")[:nb]
+    np.testing.assert_array_equal(h.lowest_diag(nb), want)
+    h.close()
+    # ties: one-electron integrals only, on the diagonal, integer valued
+    m1t = np.zeros_like(m1)
+    for i in range(n):
+        m1t[i * (i + 1) // 2 + i] = float(i % 3)
+    ht = J.Hamiltonian(sp, m1t, np.zeros_like(m2), J.DIRECT_TABLES)
+    hdt = ht.hdiag()
+    assert len(np.unique(hdt)) < len(hdt) // 4
+    np.testing.assert_array_equal(ht.lowest_diag(nb), np.argsort(hdt, kind="stable")[:nb])
+    ht.close()
+    sp.close()
+
+
 def test_user_guess_and_maxiter(J, orc, ctx):
     gs, os_, m1, m2 = _spaces(J, orc, ctx, (3, 3, 8, 0, 8, 0, 1), seed=4)
     h = J.Hamiltonian(gs, m1, m2, J.PAIRS_FP64)
