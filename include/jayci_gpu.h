@@ -333,12 +333,21 @@ int jci_batch_set_tiles(jci_batch *b, int bn64, int bn32);
 /* device bytes a slot of ndets determinants takes (basis, sigma and staging vectors) */
 long long jci_batch_bytes(long long ndets, int krymax);
 int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, int *slot);
+/* the initial block of h's next jci_batch_add with parameters p, computed now (on h's
+ * context, from any host thread that uses no other call on that context meanwhile), so that
+ * the add only uploads it; a later add with other refdim / krymin / guess settings computes
+ * its own (no counterpart in the reference: for serving loops that build requests beside
+ * the running batch) */
+int jci_ham_prepare_guess(jci_ham *h, const jci_solve_params *p);
 int jci_batch_step(jci_batch *b, int *nactive);
 int jci_batch_run(jci_batch *b, int max_steps);
 /* done: 0 running, 1 converged, 2 stopped (maxiter or no new directions) */
 int jci_batch_result(jci_batch *b, int slot, int *done, double *evals, double *evecs,
                      jci_solve_stats *stats);
 int jci_batch_release(jci_batch *b, int slot);
+/* slots whose solve has ended (converged or failed) and that are not released yet: returns
+ * their number and writes up to cap of them to slots (may be NULL), in slot order */
+int jci_batch_finished(jci_batch *b, int *slots, int cap);
 double jci_batch_sigma_seconds(const jci_batch *b);
 /* host wall seconds spent per step phase since creation: [0] sigma, [1] Gram and
  * Rayleigh-Ritz, [2] residuals and decisions, [3] selection / collapse / precision

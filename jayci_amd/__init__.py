@@ -100,6 +100,8 @@ _sig("jci_batch_step", _i, [_P, ctypes.POINTER(_i)])
 _sig("jci_batch_run", _i, [_P, _i])
 _sig("jci_batch_result", _i, [_P, _i, ctypes.POINTER(_i), _P, _P, ctypes.POINTER(SolveStats)])
 _sig("jci_batch_release", _i, [_P, _i])
+_sig("jci_batch_finished", _i, [_P, _P, _i])
+_sig("jci_ham_prepare_guess", _i, [_P, _P])
 _sig("jci_batch_sigma_seconds", _d, [_P])
 _sig("jci_batch_phase_seconds", _i, [_P, _P])
 _sig("jci_write_vectors_ufmt", _i, [ctypes.c_char_p, _i, _i, _P])
@@ -181,6 +183,7 @@ class Context:
         h = _P()
         _check(_L.jci_ctx_create(device, ctypes.byref(h)), "jci_ctx_create")
         self._h = h
+        self.device = device
 
     def close(self):
         if getattr(self, "_h", None):
@@ -301,6 +304,11 @@ class Hamiltonian:
         out = np.zeros(self.space.ndets)
         _check(_L.jci_ham_hdiag(self._h, _p(out)), "jci_ham_hdiag")
         return out
+
+    def prepare_guess(self, params):
+        """Compute now the initial block of the next Batch.add with these parameters (on
+        this Hamiltonian's context), so that the add only uploads it."""
+        _check(_L.jci_ham_prepare_guess(self._h, ctypes.byref(params)), "jci_ham_prepare_guess")
 
     def lowest_diag(self, nb):
         """Indices of the nb lowest diagonal elements, ascending by (value, index)."""
@@ -494,6 +502,7 @@ class Batch:
 
     def __init__(self, ctx, capacity, max_norbs=64, graphs=True, pool_bytes=None):
         self.ctx = ctx
+        self.capacity = capacity
         h = _P()
         _check(_L.jci_batch_create(ctx._h, capacity, max_norbs, ctypes.byref(h)), "jci_batch_create")
         self._h = h
@@ -546,6 +555,12 @@ class Batch:
                  "nsigma64": st.nsigma64, "nsigma32": st.nsigma32,
                  "rnorm": [st.rnorm[i] for i in range(params.nroots)], "seconds": st.seconds}
         return ev, vec, stats
+
+    def finished(self):
+        """Slots whose solve has ended and that are not released yet."""
+        out = np.zeros(self.capacity, dtype=np.int32)
+        n = _L.jci_batch_finished(self._h, _p(out), self.capacity)
+        return [int(x) for x in out[:min(n, self.capacity)]]
 
     def release(self, slot):
         _check(_L.jci_batch_release(self._h, slot), "jci_batch_release")

@@ -118,6 +118,9 @@ struct jci_batch {
     // device blocks of released instances, kept for reuse (oldest first)
     std::vector<std::pair<void *, size_t>> cache;
     size_t cache_bytes;
+    // device scratch of prepared initial blocks (grows only)
+    void *gscr;
+    size_t gscr_bytes;
     // host wall time per step phase: 0 sigma (all of run_sigma), 1 Gram + Rayleigh-Ritz,
     // 2 residuals and decisions, 3 selection / collapse / hand-over, 4 Gram-Schmidt and
     // orthonormalisation, 5 append
@@ -266,6 +269,7 @@ JCI_API void jci_batch_destroy(jci_batch *b)
     hipStreamSynchronize(b->c->stream);
     for (auto &s : b->in) if (s.used) free_inst(b, s);
     cache_flush(b);
+    hipFree(b->gscr);
     for (auto &g : b->graphs) hipGraphExecDestroy(g.exec);
     hipFree(b->dH); hipHostFree(b->hH);
     hipFree(b->dD);
@@ -371,7 +375,30 @@ JCI_API int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, i
     }
     s.used = 1;
     s.t_add = tnow();
-    int rc = jci_ham_guess_block(h, p->refdim, s.krymin, p->guess == JCI_GUESS_DIAG, s.V, s.ld);
+    // initial block: prepared ahead with the Hamiltonian (jci_ham_prepare_guess, any thread)
+    // it is only uploaded and scattered here, on the batch stream
+    int rc, gnb = 0;
+    const long long *gidx = nullptr;
+    const double *gU = nullptr;
+    if (jci_ham_prepared_guess(h, p->refdim, s.krymin, p->guess == JCI_GUESS_DIAG, &gnb, &gidx, &gU)) {
+        const size_t need = (sizeof(long long) + sizeof(double) * s.krymin) * (size_t)gnb;
+        rc = 0;
+        if (need > b->gscr_bytes) {
+            hipFree(b->gscr);
+            b->gscr = nullptr;
+            b->gscr_bytes = 0;
+            if (hipMalloc(&b->gscr, need) != hipSuccess) {
+                (void)hipGetLastError();
+                jci_set_error("batch: out of device memory for an initial block");
+                rc = -3;
+            } else {
+                b->gscr_bytes = need;
+            }
+        }
+        if (!rc) rc = jci_guess_apply(gnb, gidx, gU, s.krymin, s.V, s.ld, b->gscr, b->c->stream);
+    } else {
+        rc = jci_ham_guess_block(h, p->refdim, s.krymin, p->guess == JCI_GUESS_DIAG, s.V, s.ld);
+    }
     if (rc) { free_inst(b, s); s.used = 0; return rc; }
     s.m = s.krymin;
     s.sig_first = 0;
@@ -1098,6 +1125,17 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
     }
     b->phase_seconds[4] += tnow() - tp;
     return 0;
+}
+
+JCI_API int jci_batch_finished(jci_batch *b, int *slots, int cap)
+{
+    int n = 0;
+    for (int i = 0; i < b->cap; i++)
+        if (b->in[i].used && b->in[i].done) {
+            if (slots && n < cap) slots[n] = i;
+            n++;
+        }
+    return n;
 }
 
 JCI_API int jci_batch_result(jci_batch *b, int slot, int *done, double *evals, double *evecs,

@@ -105,7 +105,7 @@ static void ctx_free(jci_ctx *c)
 JCI_API void jci_ctx_destroy(jci_ctx *c)
 {
     if (!c) return;
-    if (c->refs > 0) { c->closing = 1; return; }
+    if (__atomic_load_n(&c->refs, __ATOMIC_ACQUIRE) > 0) { c->closing = 1; return; }
     ctx_free(c);
 }
 
@@ -144,7 +144,8 @@ JCI_API int jci_space_create(jci_ctx *ctx, int aelec, int belec, int norbs, int 
         RCHECK(jk_gen_strings(s->d_desc, s->h, sp, ctx->d_binom, s->d_mask[sp], ctx->stream));
     }
     JCI_CHECK(hipStreamSynchronize(ctx->stream));
-    ctx->refs++;
+    // contexts may create spaces on one host thread and free them on another
+    __atomic_add_fetch(&ctx->refs, 1, __ATOMIC_ACQ_REL);
     *out = s;
     return 0;
 }
@@ -158,7 +159,7 @@ static void space_free(jci_space *s)
     hipFree(s->d_mask[1]);
     jci_ctx *c = s->ctx;
     free(s);
-    if (--c->refs == 0 && c->closing) ctx_free(c);
+    if (__atomic_sub_fetch(&c->refs, 1, __ATOMIC_ACQ_REL) == 0 && c->closing) ctx_free(c);
 }
 
 // a space is freed when it was destroyed and no Hamiltonian refers to it any more
@@ -284,6 +285,11 @@ struct jci_ham {
     GfHeader *h_gfh, *d_gfh;
     int gf_valid[JCI_MAXK];
     int gf_bn64, gf_bn32;       // E tile widths (autotuned)
+    // initial block computed ahead (jci_ham_prepare_guess): its determinants and the lowest
+    // g_nguess eigenvectors of H on them (vector j at g_U + j*g_nb), host memory
+    long long *g_idx;
+    double *g_U;
+    int g_nb, g_nguess, g_refdim, g_diag;
 };
 
 static size_t m1_len(int n) { return (size_t)n * (n + 1) / 2; }
@@ -377,16 +383,17 @@ static int build_gf(jci_ham *h)
         // alpha lists with the E row of each entry (the pair's position in the target's list)
         uint32_t *raw = nullptr;
         int *d_missing = nullptr, missing = 0;
-        JCI_CHECK(hipMalloc(&raw, sizeof(uint32_t) * na * La));
-        JCI_CHECK(hipMalloc(&d_missing, sizeof(int)));
+        // temporaries stream-ordered: a hipFree would wait for every stream of the device
+        JCI_CHECK(hipMallocAsync((void **)&raw, sizeof(uint32_t) * na * La, c->stream));
+        JCI_CHECK(hipMallocAsync((void **)&d_missing, sizeof(int), c->stream));
         JCI_CHECK(hipMemsetAsync(d_missing, 0, sizeof(int), c->stream));
         int rc = jk_gf_lists(na, s->d_mask[0], h->d_off1[0], h->d_e1[0], La, 1, raw, c->stream);
         if (!rc) rc = jk_gf_rank(na, La, raw, h->d_gfla, d_missing, c->stream);
         if (!rc && hipMemcpyAsync(&missing, d_missing, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
             rc = -2;
         if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = -2;
-        hipFree(raw);
-        hipFree(d_missing);
+        hipFreeAsync(raw, c->stream);
+        hipFreeAsync(d_missing, c->stream);
         if (rc) return rc;
         if (missing) {
             jci_set_error("gf: %d alpha-list entries without a row in their target's list", missing);
@@ -428,8 +435,8 @@ static int build_fast(jci_ham *h, int W)
         long long n = D.s[spin].gstart[D.s[spin].ngrp];
         long long *c1, *c2;
         int G = D.s[spin].ngrp;
-        JCI_CHECK(hipMalloc(&c1, sizeof(long long) * (n + 1)));
-        JCI_CHECK(hipMalloc(&c2, sizeof(long long) * (n + 1)));
+        JCI_CHECK(hipMallocAsync((void **)&c1, sizeof(long long) * (n + 1), c->stream));
+        JCI_CHECK(hipMallocAsync((void **)&c2, sizeof(long long) * (n + 1), c->stream));
         JCI_CHECK(hipMalloc(&h->d_off1[spin], sizeof(long long) * (n + 1)));
         JCI_CHECK(hipMalloc(&h->d_off2[spin], sizeof(long long) * (n + 1)));
         JCI_CHECK(hipMalloc(&h->d_goff1[spin], sizeof(int) * n * (G + 1)));
@@ -443,8 +450,8 @@ static int build_fast(jci_ham *h, int W)
         JCI_CHECK(hipMemcpyAsync(&h->nexc2[spin], h->d_off2[spin] + n, sizeof(long long),
                                  hipMemcpyDeviceToHost, c->stream));
         JCI_CHECK(hipStreamSynchronize(c->stream));
-        hipFree(c1);
-        hipFree(c2);
+        hipFreeAsync(c1, c->stream);
+        hipFreeAsync(c2, c->stream);
         size_t a1 = h->nexc1[spin] > 0 ? h->nexc1[spin] : 1, a2 = h->nexc2[spin] > 0 ? h->nexc2[spin] : 1;
         if (hipMalloc(&h->d_e1[spin], sizeof(int2) * a1) != hipSuccess ||
             hipMalloc(&h->d_v1[spin], sizeof(double) * a1) != hipSuccess ||
@@ -570,6 +577,8 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     hipFree(h->d_gft); hipFree(h->d_gfh);
     if (h->h_gft) hipHostFree(h->h_gft);
     if (h->h_gfh) hipHostFree(h->h_gfh);
+    free(h->g_idx);
+    free(h->g_U);
     jci_space *s = h->sp;
     free(h);
     if (--s->refs == 0 && s->closing) space_free(s);
@@ -1166,7 +1175,7 @@ static int lowest_diag(jci_ctx *c, const double *d_hd, long long N, int nb, long
     if (nb <= JCI_GUESS_CHUNK) {
         const size_t ncand = (size_t)((N + JCI_GUESS_CHUNK - 1) / JCI_GUESS_CHUNK) * nb;
         void *cand = nullptr;
-        JCI_CHECK(hipMalloc(&cand, (sizeof(double) + sizeof(long long)) * ncand));
+        JCI_CHECK(hipMallocAsync(&cand, (sizeof(double) + sizeof(long long)) * ncand, c->stream));
         double *d_cv = (double *)cand;
         long long *d_ci = (long long *)(d_cv + ncand);
         hv.resize(ncand);
@@ -1178,7 +1187,7 @@ static int lowest_diag(jci_ctx *c, const double *d_hd, long long N, int nb, long
                                    c->stream) != hipSuccess ||
                     hipStreamSynchronize(c->stream) != hipSuccess))
             rc = -2;
-        hipFree(cand);
+        hipFreeAsync(cand, c->stream);
         if (rc) return rc;
     } else {
         hv.resize(N);
@@ -1202,43 +1211,82 @@ JCI_API int jci_ham_lowest_diag(jci_ham *h, int nb, long long *idx)
     return lowest_diag(h->sp->ctx, h->d_hd, h->sp->h.ndets, nb, idx);
 }
 
-static int guess_block(Work &w, const jci_solve_params *p, int nguess)
+// the initial block: its nb determinants (the lowest diagonal elements, or the leading
+// ones) and the lowest nguess eigenvectors of H on them (vector j at U + j*nb)
+static int guess_compute(SolveOp *op, jci_ctx *c, const jci_solve_params *p, int nguess,
+                         std::vector<long long> &idx, std::vector<double> &U)
 {
-    SolveOp *op = w.op;
-    long long N = w.N;
+    const long long N = op->N;
     int nb = (int)std::min<long long>(p->refdim, N);
     if (nb < nguess) nb = (int)std::min<long long>(nguess, N);
-    std::vector<long long> idx(nb);
+    idx.assign(nb, 0);
     if (p->guess == JCI_GUESS_DIAG) {
-        RCHECK(lowest_diag(w.c, op->d_hd, N, nb, idx.data()));
+        RCHECK(lowest_diag(c, op->d_hd, N, nb, idx.data()));
     } else {
         for (int i = 0; i < nb; i++) idx[i] = i;
     }
-    // device scratch in one block: indices, masks, the block of H and its eigenvectors
+    // device scratch in one block: indices, masks, the block of H
     void *scratch = nullptr;
-    JCI_CHECK(hipMalloc(&scratch, sizeof(long long) * nb + sizeof(uint64_t) * 2 * nb +
-                                      sizeof(double) * ((size_t)nb * nb + (size_t)nb * nguess)));
+    JCI_CHECK(hipMallocAsync(&scratch, sizeof(long long) * nb + sizeof(uint64_t) * 2 * nb +
+                                           sizeof(double) * (size_t)nb * nb, c->stream));
     long long *d_idx = (long long *)scratch;
     uint64_t *d_m = (uint64_t *)(d_idx + nb);
     double *d_H = (double *)(d_m + 2 * nb);
-    double *d_U = d_H + (size_t)nb * nb;
-    JCI_CHECK(hipMemcpyAsync(d_idx, idx.data(), sizeof(long long) * nb, hipMemcpyHostToDevice,
-                             w.c->stream));
-    RCHECK(op->masks(op->self, nb, d_idx, d_m, d_m + nb));
-    RCHECK(jk_dense_block(nb, d_m, d_m + nb, 0, op->d_m1, op->d_m2, d_H, w.c->stream));
-    std::vector<double> H((size_t)nb * nb), ev(nb), U((size_t)nb * nb);
-    JCI_CHECK(hipMemcpyAsync(H.data(), d_H, sizeof(double) * nb * nb, hipMemcpyDeviceToHost,
-                             w.c->stream));
-    JCI_CHECK(hipStreamSynchronize(w.c->stream));
-    jacobi_eig(nb, H.data(), ev.data(), U.data());
-    // first nguess eigenvectors of the block, scattered to their determinants
-    JCI_CHECK(hipMemcpyAsync(d_U, U.data(), sizeof(double) * nb * nguess, hipMemcpyHostToDevice,
-                             w.c->stream));
-    JCI_CHECK(hipMemsetAsync(w.V, 0, sizeof(double) * w.ld * nguess, w.c->stream));
-    RCHECK(jk_scatter_guess(nb, nguess, d_idx, d_U, w.V, w.ld, w.c->stream));
-    JCI_CHECK(hipStreamSynchronize(w.c->stream));
-    hipFree(scratch);
+    std::vector<double> H((size_t)nb * nb), ev(nb), Uf((size_t)nb * nb);
+    int rc = 0;
+    if (hipMemcpyAsync(d_idx, idx.data(), sizeof(long long) * nb, hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        rc = -2;
+    if (!rc) rc = op->masks(op->self, nb, d_idx, d_m, d_m + nb);
+    if (!rc) rc = jk_dense_block(nb, d_m, d_m + nb, 0, op->d_m1, op->d_m2, d_H, c->stream);
+    if (!rc && (hipMemcpyAsync(H.data(), d_H, sizeof(double) * nb * nb, hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = -2;
+    hipFreeAsync(scratch, c->stream);
+    if (rc) return rc;
+    jacobi_eig(nb, H.data(), ev.data(), Uf.data());
+    U.assign(Uf.begin(), Uf.begin() + (size_t)nb * nguess);
     return 0;
+}
+
+// V[:, 0..nguess) = 0, then the guess vectors scattered to their determinants, on stream st;
+// d_scr holds (8 + 8 nguess) nb bytes
+static int guess_apply(const std::vector<long long> &idx, const std::vector<double> &U,
+                       int nguess, double *V, long long ld, void *d_scr, hipStream_t st)
+{
+    const int nb = (int)idx.size();
+    long long *d_idx = (long long *)d_scr;
+    double *d_U = (double *)(d_idx + nb);
+    JCI_CHECK(hipMemcpyAsync(d_idx, idx.data(), sizeof(long long) * nb, hipMemcpyHostToDevice, st));
+    JCI_CHECK(hipMemcpyAsync(d_U, U.data(), sizeof(double) * nb * nguess, hipMemcpyHostToDevice, st));
+    JCI_CHECK(hipMemsetAsync(V, 0, sizeof(double) * ld * nguess, st));
+    RCHECK(jk_scatter_guess(nb, nguess, d_idx, d_U, V, ld, st));
+    return 0;
+}
+
+static int guess_block(Work &w, const jci_solve_params *p, int nguess)
+{
+    std::vector<long long> idx;
+    std::vector<double> U;
+    RCHECK(guess_compute(w.op, w.c, p, nguess, idx, U));
+    void *scr = nullptr;
+    JCI_CHECK(hipMalloc(&scr, (sizeof(long long) + sizeof(double) * nguess) * idx.size()));
+    int rc = guess_apply(idx, U, nguess, w.V, w.ld, scr, w.c->stream);
+    if (!rc && hipStreamSynchronize(w.c->stream) != hipSuccess) rc = -2;
+    hipFree(scr);
+    return rc;
+}
+
+void jci_subspace_sizes(const jci_solve_params *p, long long N, int *ntrack, int *krymin,
+                        int *krymax)
+{
+    const int kmax = (int)std::min<long long>(p->krymax, N);
+    const int nt = (int)std::min(std::min<long long>(p->nroots + p->nbuffer, JCI_MAXROOTS),
+                                 std::min<long long>(N, std::max<long long>(p->nroots, kmax / 2)));
+    *krymax = kmax;
+    *ntrack = nt;
+    *krymin = std::min(std::max(p->krymin, nt), kmax);
 }
 
 int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, double *evals,
@@ -1573,6 +1621,61 @@ int jci_ham_fastargs(jci_ham *h, FastArgs *out)
 const double *jci_ham_hd(jci_ham *h) { return h->d_hd; }
 long long jci_ham_ndets(jci_ham *h) { return h->sp->h.ndets; }
 jci_ctx *jci_ham_ctx(jci_ham *h) { return h->sp->ctx; }
+
+// the initial block of the Hamiltonian's next batched solve, computed ahead (any host thread,
+// on the Hamiltonian's context) so that jci_batch_add only uploads and scatters it
+JCI_API int jci_ham_prepare_guess(jci_ham *h, const jci_solve_params *p)
+{
+    JCI_CHECK(hipSetDevice(h->sp->ctx->device));
+    SolveOp op{};
+    op.self = h;
+    op.ctx = h->sp->ctx;
+    op.N = h->sp->h.ndets;
+    op.d_hd = h->d_hd;
+    op.d_m1 = h->d_m1;
+    op.d_m2 = h->d_m2;
+    op.masks = ham_masks;
+    int ntrack, krymin, krymax;
+    jci_subspace_sizes(p, op.N, &ntrack, &krymin, &krymax);
+    jci_solve_params q = *p;
+    q.guess = p->guess == JCI_GUESS_DIAG ? JCI_GUESS_DIAG : JCI_GUESS_BLOCK;
+    std::vector<long long> idx;
+    std::vector<double> U;
+    RCHECK(guess_compute(&op, op.ctx, &q, krymin, idx, U));
+    free(h->g_idx);
+    free(h->g_U);
+    h->g_idx = (long long *)malloc(sizeof(long long) * idx.size());
+    h->g_U = (double *)malloc(sizeof(double) * U.size());
+    if (!h->g_idx || !h->g_U) { jci_set_error("prepare_guess: out of host memory"); return -3; }
+    memcpy(h->g_idx, idx.data(), sizeof(long long) * idx.size());
+    memcpy(h->g_U, U.data(), sizeof(double) * U.size());
+    h->g_nb = (int)idx.size();
+    h->g_nguess = krymin;
+    h->g_refdim = p->refdim;
+    h->g_diag = q.guess == JCI_GUESS_DIAG;
+    return 0;
+}
+
+// a prepared initial block for (refdim, nguess, diag): its size nb and host arrays, or 0
+int jci_ham_prepared_guess(jci_ham *h, int refdim, int nguess, int diag_guess, int *nb,
+                           const long long **idx, const double **U)
+{
+    if (!h->g_idx || h->g_refdim != refdim || h->g_nguess != nguess || h->g_diag != (diag_guess != 0))
+        return 0;
+    *nb = h->g_nb;
+    *idx = h->g_idx;
+    *U = h->g_U;
+    return 1;
+}
+
+// the initial block applied to V on stream st; d_scr holds (8 + 8 nguess) nb bytes
+int jci_guess_apply(int nb, const long long *idx, const double *U, int nguess, double *V,
+                    long long ld, void *d_scr, hipStream_t st)
+{
+    std::vector<long long> vi(idx, idx + nb);
+    std::vector<double> vu(U, U + (size_t)nb * nguess);
+    return guess_apply(vi, vu, nguess, V, ld, d_scr, st);
+}
 
 // initial guess block of a Hamiltonian into V (nguess vectors, leading dimension ld)
 int jci_ham_guess_block(jci_ham *h, int refdim, int nguess, int diag_guess, double *V, long long ld)

@@ -90,6 +90,18 @@ long long jci_ham_ndets(jci_ham *h);
 // (kmax x kmax) C^T G C row-major (cols x cols).  Returns cols.
 int jci_gdk_restart(int m, const double *G, const double *Yf, int kc, int mp, int np,
                     const double *Yp, int kmax, double *C, double *Gc);
+// subspace sizes of a solve: tracked pairs (roots + buffer pairs, space permitting), the
+// restart size krymin (also the initial block) and krymax, for N determinants
+void jci_subspace_sizes(const jci_solve_params *p, long long N, int *ntrack, int *krymin,
+                        int *krymax);
+// a Hamiltonian's prepared initial block (jci_ham_prepare_guess) for these settings: 1 and its
+// size and host arrays (U: vector j at U + j*nb), or 0
+int jci_ham_prepared_guess(jci_ham *h, int refdim, int nguess, int diag_guess, int *nb,
+                           const long long **idx, const double **U);
+// V[:, 0..nguess) = the block's vectors scattered to their determinants, on stream st; d_scr
+// holds (8 + 8 nguess) nb bytes
+int jci_guess_apply(int nb, const long long *idx, const double *U, int nguess, double *V,
+                    long long ld, void *d_scr, hipStream_t st);
 // jci_solve returning the lowest max(nroots, nout) Ritz vectors
 int jci_solve_nout(jci_ham *h, const jci_solve_params *p, const double *guess, double *evals,
                    double *evecs, jci_solve_stats *stats, int nout);

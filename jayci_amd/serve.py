@@ -13,11 +13,12 @@ alpha = beta electrons ~ U{3..5}, roots ~ U{1..4}); arrival gaps are exponential
 """
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import (Batch, Hamiltonian, Space, DIRECT_TABLES, GUESS_DIAG, PREC_FP64, RESTART_GDK,
-               SYNTH_ENUC, synth_integrals)
+from . import (Batch, Context, Hamiltonian, Space, DIRECT_TABLES, GUESS_DIAG, PREC_FP64,
+               RESTART_GDK, SYNTH_ENUC, synth_integrals)
 
 
 def instance_spec(rng):
@@ -62,7 +63,7 @@ def solve_params(spec, N, restol=1e-6, nbuffer=1, etol=1e-9, kfac=8, buftol=1e-3
 
 class Request:
     __slots__ = ("idx", "arrival", "spec", "seed", "space", "ham", "slot", "t_admit", "t_done",
-                 "energies", "stats", "bytes")
+                 "energies", "stats", "bytes", "params", "future")
 
 
 def percentile(xs, q):
@@ -87,9 +88,24 @@ class Server:
         self.mixed_switch = mixed_switch
         self.nbuffer = nbuffer
         self.batch = Batch(ctx, capacity, max_norbs, graphs=graphs, pool_bytes=pool_bytes)
+        # requests are built (integrals, strings, Hamiltonian, initial block) on a worker
+        # thread with a context of its own, beside the steps of the running batch
+        self.bctx = Context(ctx.device)
+        self.builder = ThreadPoolExecutor(max_workers=1)
 
     def close(self):
+        self.builder.shutdown(wait=True)
         self.batch.close()
+        self.bctx.close()
+
+    def _build(self, r):
+        s = r.spec
+        m1, m2 = synth_integrals(s["norbs"], r.seed)
+        r.space = Space(self.bctx, s["na"], s["nb"], s["norbs"], s["ndocc"], s["nactv"],
+                        s["xlvl"], s["rule"])
+        r.ham = Hamiltonian(r.space, m1, m2, DIRECT_TABLES)
+        r.params = self._params(r)
+        r.ham.prepare_guess(r.params)
 
     def _params(self, req):
         sp = req.spec
@@ -114,26 +130,29 @@ class Server:
             r.t_admit = r.t_done = None
             r.bytes = self._estimate_bytes(spec)
             reqs.append(r)
-        waiting, active, done = [], {}, []
+        waiting, building, active, done = [], [], {}, []
         used = 0.0
         nxt = 0
         t0 = time.perf_counter()
-        while nxt < len(reqs) or waiting or active:
+        while nxt < len(reqs) or waiting or building or active:
             now = time.perf_counter() - t0
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
                 waiting.append(reqs[nxt])
                 nxt += 1
-            # admission: first come first served, within slots and HBM budget
-            while waiting and len(active) < self.capacity and used + waiting[0].bytes <= self.budget:
+            # admission, first come first served within slots and HBM budget: the request is
+            # built on the worker, then added to the batch here once its build has finished
+            # (waiting for it only when nothing else could step)
+            while (waiting and len(active) + len(building) < self.capacity
+                   and used + waiting[0].bytes <= self.budget):
                 r = waiting.pop(0)
-                s = r.spec
-                m1, m2 = synth_integrals(s["norbs"], r.seed)
-                r.space = Space(self.ctx, s["na"], s["nb"], s["norbs"], s["ndocc"], s["nactv"],
-                                s["xlvl"], s["rule"])
-                r.ham = Hamiltonian(r.space, m1, m2, DIRECT_TABLES)
-                r.slot = self.batch.add(r.ham, self._params(r))
-                r.t_admit = time.perf_counter() - t0
                 used += r.bytes
+                r.future = self.builder.submit(self._build, r)
+                building.append(r)
+            while building and (building[0].future.done() or not active):
+                r = building.pop(0)
+                r.future.result()
+                r.slot = self.batch.add(r.ham, r.params)
+                r.t_admit = time.perf_counter() - t0
                 active[r.slot] = r
             if not active:
                 if nxt < len(reqs):
@@ -141,9 +160,9 @@ class Server:
                 continue
             self.batch.step()
             tnow = time.perf_counter() - t0
-            for slot in list(active):
-                ev, _, st = self.batch.result(slot)
-                if st["done"]:
+            for slot in self.batch.finished():
+                if slot in active:
+                    ev, _, st = self.batch.result(slot)
                     r = active.pop(slot)
                     r.t_done = tnow
                     r.energies = ev
