@@ -155,6 +155,10 @@ class Server:
                 r.t_admit = time.perf_counter() - t0
                 active[r.slot] = r
             if not active:
+                if waiting and not building:
+                    # nothing runs and the head of the queue still does not fit
+                    raise RuntimeError("request %d needs %.3g bytes, above the budget of %.3g"
+                                       % (waiting[0].idx, waiting[0].bytes, self.budget))
                 if nxt < len(reqs):
                     time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - now)))
                 continue

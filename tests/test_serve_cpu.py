@@ -1,0 +1,36 @@
+"""Serving loop host logic (no GPU): solver settings of a request, the seeded stream, and the
+admission error for a request that cannot fit the budget."""
+import numpy as np
+import pytest
+
+
+def test_solve_params_served_settings(J):
+    from jayci_amd import serve
+    spec = {"nroots": 3}
+    kw = serve.solve_params(spec, 4008004)
+    assert kw["nbuffer"] == 1 and kw["buftol"] == 1e-3 and kw["restart"] == J.RESTART_GDK
+    assert kw["krymin"] == 5 and kw["krymax"] == 32 and kw["etol"] == 1e-9
+    small = serve.solve_params(spec, 10)
+    assert small["krymax"] == 10 and small["refdim"] == 10
+    p = J.make_params(10, **{k: v for k, v in small.items() if k != "nroots"}, nroots=3)
+    assert p.buftol == 1e-3 and p.restart == J.RESTART_GDK
+
+
+def test_stream_is_seeded():
+    from jayci_amd import serve
+    a = serve.make_stream(16, 200.0, 400)
+    b = serve.make_stream(16, 200.0, 400)
+    assert a == b
+    t = [x[0] for x in a]
+    assert all(t2 > t1 for t1, t2 in zip(t, t[1:]))
+    for _, spec, _ in a:
+        assert 8 <= spec["norbs"] <= 14 and 3 <= spec["na"] <= 5 and 1 <= spec["nroots"] <= 4
+
+
+def test_request_above_budget_is_an_error():
+    from jayci_amd import serve
+    srv = serve.Server.__new__(serve.Server)   # host state only: no batch, no context
+    srv.capacity, srv.budget, srv.restol, srv.nbuffer = 4, 1.0, 1e-6, 1
+    stream = [(0.0, serve.instance_spec(np.random.default_rng(1)), 1)]
+    with pytest.raises(RuntimeError, match="above the budget"):
+        srv.run(stream)
