@@ -512,6 +512,7 @@ static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vec
     if (n == 0) return 0;
     if (n > b->desc_cap) { jci_set_error("batch: %d descriptors above %d", n, b->desc_cap); return -1; }
     long long blk = 0;
+    int mmax = 1;
     for (int t = 0; t < n; t++) {
         BInst &s = b->in[ids[t]];
         BDesc &d = b->hD[t];
@@ -520,11 +521,19 @@ static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vec
         d.C = b->dcoef + cofs[t]; d.beta = beta;
         d.blk0 = blk; d.nblk = nblk_rows(s.N);
         blk += d.nblk;
+        mmax = std::max(mmax, m[t]);
+        if (k[t] > m[t]) {
+            jci_set_error("batch: combine of %d vectors into %d", m[t], k[t]);
+            return -1;
+        }
     }
     RCHECK(upload_coef(b, coef_len));
     RCHECK(upload_desc(b, n, blk));
     RCHECK(stage_done(b));
-    RCHECK(jb_combine(b->dH, b->dD, b->c->stream));
+    if (jb_combine(b->dH, b->dD, mmax, b->c->stream)) {
+        jci_set_error("batch: combine of %d basis vectors above %d", mmax, JCI_MAXBASIS);
+        return -1;
+    }
     return 0;
 }
 

@@ -42,7 +42,8 @@ struct BSigma {
 
 // all launches use fixed grids; n_cap bounds the instance count (reduction grids)
 int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, double *partial, hipStream_t st);
-int jb_combine(const BHeader *h, const BDesc *dD, hipStream_t st);
+// mmax: the largest m of the launch's descriptors
+int jb_combine(const BHeader *h, const BDesc *dD, int mmax, hipStream_t st);
 int jb_residual(const BHeader *h, const BDesc *dD, int n_cap, double *partial, hipStream_t st);
 int jb_pack(const BHeader *h, const BDesc *dD, int pack, hipStream_t st);
 int jb_to32(const BHeader *h, const BDesc *dD, hipStream_t st);

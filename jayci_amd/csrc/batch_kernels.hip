@@ -139,37 +139,53 @@ int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, double *partial, hipSt
 }
 
 // ---- combine: O_j = beta * O_j + sum_i A_i C[i*k + j] (in place when O == A) -----------
+// MAXM bounds m over the launch (the host picks the smallest instance that holds it): a
+// thread keeps the m inputs of its row in registers, so every input of the row is read before
+// any output is written (in place is safe), and the register array is only as large as the
+// launch needs.  The coefficients are staged in LDS when the instance changes; every
+// descriptor has k <= m.
+template <int MAXM>
 __global__ __launch_bounds__(256) void k_bcombine(const BHeader *__restrict__ H,
                                                   const BDesc *__restrict__ D)
 {
-    __shared__ double sC[JCI_MAXBASIS * JCI_MAXBASIS];
+    __shared__ double sC[MAXM * MAXM];   // k <= m <= MAXM (checked by the host)
     const int n = H->n;
     const long long total = H->total;
+    int cur = -1;
     for (long long blk = blockIdx.x; blk < total; blk += gridDim.x) {
-    const int inst = find_inst(D, n, blk);
-    const BDesc &d = D[inst];
-    __syncthreads();
-    for (int t = threadIdx.x; t < d.m * d.k; t += blockDim.x) sC[t] = d.C[t];
-    __syncthreads();
-    long long r = (long long)(blk - d.blk0) * 256 + threadIdx.x;
-    if (r >= d.N) continue;
-    double a[JCI_MAXBASIS];
+        const int inst = find_inst(D, n, blk);
+        const BDesc &d = D[inst];
+        if (inst != cur) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < d.m * d.k; t += blockDim.x) sC[t] = d.C[t];
+            __syncthreads();
+            cur = inst;
+        }
+        const long long r = (long long)(blk - d.blk0) * 256 + threadIdx.x;
+        if (r >= d.N) continue;
+        double a[MAXM];
 #pragma unroll
-    for (int i = 0; i < JCI_MAXBASIS; i++)
-        if (i < d.m) a[i] = d.A[(long long)i * d.ld + r];
-    for (int j = 0; j < d.k; j++) {
-        double s = d.beta == 0.0 ? 0.0 : d.beta * d.O[(long long)j * d.ld + r];
+        for (int i = 0; i < MAXM; i++)
+            if (i < d.m) a[i] = d.A[(long long)i * d.ld + r];
+        for (int j = 0; j < d.k; j++) {
+            double s = d.beta == 0.0 ? 0.0 : d.beta * d.O[(long long)j * d.ld + r];
 #pragma unroll
-        for (int i = 0; i < JCI_MAXBASIS; i++)
-            if (i < d.m) s += a[i] * sC[i * d.k + j];
-        d.O[(long long)j * d.ld + r] = s;
-    }
+            for (int i = 0; i < MAXM; i++)
+                if (i < d.m) s += a[i] * sC[i * d.k + j];
+            d.O[(long long)j * d.ld + r] = s;
+        }
     }
 }
 
-int jb_combine(const BHeader *h, const BDesc *dD, hipStream_t st)
+int jb_combine(const BHeader *h, const BDesc *dD, int mmax, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bcombine, dim3(JB_GRID), dim3(256), 0, st, h, dD);
+    if (mmax > JCI_MAXBASIS) return -1;
+    if (mmax <= 16)
+        hipLaunchKernelGGL(k_bcombine<16>, dim3(JB_GRID), dim3(256), 0, st, h, dD);
+    else if (mmax <= 32)
+        hipLaunchKernelGGL(k_bcombine<32>, dim3(JB_GRID), dim3(256), 0, st, h, dD);
+    else
+        hipLaunchKernelGGL(k_bcombine<JCI_MAXBASIS>, dim3(JB_GRID), dim3(256), 0, st, h, dD);
     BLAUNCH_CHECK();
     return 0;
 }
