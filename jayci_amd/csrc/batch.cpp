@@ -479,11 +479,17 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
     RCHECK(stage_next(b));
     long long blk = 0, pofs = 0;
     size_t o = 0;
+    int mmax = 1;
     ofs.assign(n, 0);
     for (int t = 0; t < n; t++) {
         BInst &s = b->in[ids[t]];
         BDesc &d = b->hD[t];
         memset(&d, 0, sizeof(d));
+        if (m[t] < 1 || w[t] < 1 || m[t] > JCI_MAXBASIS || w[t] > JCI_MAXBASIS) {
+            jci_set_error("batch: Gram of %d x %d outside 1..%d", m[t], w[t], JCI_MAXBASIS);
+            return -1;
+        }
+        mmax = std::max(mmax, std::max(m[t], w[t]));
         d.N = s.N; d.ld = s.ld; d.A = A[t]; d.B = B[t]; d.m = m[t]; d.k = w[t];
         d.blk0 = blk; d.nblk = nblk_red(s.N); d.pofs = pofs;
         d.out = b->dout + o;
@@ -496,7 +502,7 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
     RCHECK(ensure_partial(b, (size_t)pofs));
     RCHECK(upload_desc(b, n, blk));
     RCHECK(stage_done(b));
-    RCHECK(jb_gram(b->dH, b->dD, n, b->dpartial, b->c->stream));
+    RCHECK(jb_gram(b->dH, b->dD, n, mmax, b->dpartial, b->c->stream));
     RCHECK(download_out(b, o));
     return 0;
 }

@@ -41,7 +41,9 @@ struct BSigma {
 };
 
 // all launches use fixed grids; n_cap bounds the instance count (reduction grids)
-int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, double *partial, hipStream_t st);
+// MAXM-sized Gram launch: mmax bounds every descriptor's m and k (<= JCI_MAXBASIS)
+int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, int mmax, double *partial,
+            hipStream_t st);
 // mmax: the largest m of the launch's descriptors
 int jb_combine(const BHeader *h, const BDesc *dD, int mmax, hipStream_t st);
 int jb_residual(const BHeader *h, const BDesc *dD, int n_cap, double *partial, hipStream_t st);

@@ -34,13 +34,17 @@ __device__ __forceinline__ int find_inst(const BDesc *__restrict__ d, int n, lon
 
 // ---- Gram: out = A^T B (m x k), A and B vector-major with leading dimension ld ----------
 // FP64 MFMA 16x16x4 on 32-row LDS tiles (operand layout as k_gram); partial per block.
+// MAXM bounds m and k over the launch (checked by the host): the LDS tiles, the staging
+// registers and the (MAXM/16)^2 output tiles spread over the four waves are sized to it.
 #define BG_ROWS 32
+template <int MAXM>
 __global__ __launch_bounds__(256) void k_bgram(const BHeader *__restrict__ H,
                                                const BDesc *__restrict__ D,
                                                double *__restrict__ partial)
 {
-    __shared__ double sA[BG_ROWS][JCI_MAXBASIS + 1];
-    __shared__ double sB[BG_ROWS][JCI_MAXBASIS + 1];
+    constexpr int NT = ((MAXM / 16) * (MAXM / 16) + 3) / 4;   // output tiles per wave
+    __shared__ double sA[BG_ROWS][MAXM + 1];
+    __shared__ double sB[BG_ROWS][MAXM + 1];
     const int n = H->n;
     const long long total = H->total;
     for (long long blk = blockIdx.x; blk < total; blk += gridDim.x) {
@@ -55,12 +59,12 @@ __global__ __launch_bounds__(256) void k_bgram(const BHeader *__restrict__ H,
     const long long r0 = (long long)b * chunk;
     const long long r1 = r0 + chunk < d.N ? r0 + chunk : d.N;
     typedef double v4d __attribute__((ext_vector_type(4)));
-    v4d acc[4];
-    for (int t = 0; t < 4; t++) acc[t] = (v4d){0.0, 0.0, 0.0, 0.0};
+    v4d acc[NT];
+    for (int t = 0; t < NT; t++) acc[t] = (v4d){0.0, 0.0, 0.0, 0.0};
     // the chunk's values this thread stages (element idx = tid + 256 q: column idx / BG_ROWS,
     // row idx % BG_ROWS), loaded into registers one chunk ahead so that the loads of the
     // next chunk are in flight during the products of this one
-    constexpr int QA = BG_ROWS * JCI_MAXBASIS / 256, QB = BG_ROWS * JCI_MAXBASIS / 256;
+    constexpr int QA = BG_ROWS * MAXM / 256, QB = BG_ROWS * MAXM / 256;
     double ra[QA], rb[QB];
     auto gload = [&](long long base) {
         const int rows = (int)((r1 - base) < BG_ROWS ? (r1 - base) : BG_ROWS);
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(256) void k_bgram(const BHeader *__restrict__ H,
         }
         __syncthreads();
         if (base + BG_ROWS < r1) gload(base + BG_ROWS);
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < NT; t++) {
             int tile = wave + 4 * t;
             if (tile >= ntile) break;
             int ti = tile / wt, tj = tile % wt;
@@ -102,7 +106,7 @@ __global__ __launch_bounds__(256) void k_bgram(const BHeader *__restrict__ H,
         __syncthreads();
     }
     double *out = partial + d.pofs + (long long)b * m * w;
-    for (int t = 0; t < 4; t++) {
+    for (int t = 0; t < NT; t++) {
         int tile = wave + 4 * t;
         if (tile >= ntile) break;
         int ti = tile / wt, tj = tile % wt;
@@ -129,9 +133,16 @@ __global__ void k_bpartial_sum(const BHeader *__restrict__ H, const BDesc *__res
     }
 }
 
-int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, double *partial, hipStream_t st)
+int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, int mmax, double *partial,
+            hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bgram, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
+    if (mmax > JCI_MAXBASIS) return -1;
+    if (mmax <= 16)
+        hipLaunchKernelGGL(k_bgram<16>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
+    else if (mmax <= 32)
+        hipLaunchKernelGGL(k_bgram<32>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
+    else
+        hipLaunchKernelGGL(k_bgram<JCI_MAXBASIS>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
     hipLaunchKernelGGL(k_bpartial_sum, dim3(n_cap), dim3(256), 0, st, h, dD,
                        (const double *)partial);
     BLAUNCH_CHECK();
