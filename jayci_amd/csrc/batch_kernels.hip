@@ -141,6 +141,8 @@ int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, int mmax, double *part
         hipLaunchKernelGGL(k_bgram<16>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
     else if (mmax <= 32)
         hipLaunchKernelGGL(k_bgram<32>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
+    else if (mmax <= 48)
+        hipLaunchKernelGGL(k_bgram<48>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
     else
         hipLaunchKernelGGL(k_bgram<JCI_MAXBASIS>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
     hipLaunchKernelGGL(k_bpartial_sum, dim3(n_cap), dim3(256), 0, st, h, dD,
@@ -195,6 +197,8 @@ int jb_combine(const BHeader *h, const BDesc *dD, int mmax, hipStream_t st)
         hipLaunchKernelGGL(k_bcombine<16>, dim3(JB_GRID), dim3(256), 0, st, h, dD);
     else if (mmax <= 32)
         hipLaunchKernelGGL(k_bcombine<32>, dim3(JB_GRID), dim3(256), 0, st, h, dD);
+    else if (mmax <= 48)
+        hipLaunchKernelGGL(k_bcombine<48>, dim3(JB_GRID), dim3(256), 0, st, h, dD);
     else
         hipLaunchKernelGGL(k_bcombine<JCI_MAXBASIS>, dim3(JB_GRID), dim3(256), 0, st, h, dD);
     BLAUNCH_CHECK();

@@ -141,3 +141,29 @@ def test_batch_pool_rounds_and_buffer_roots(J, orc, ctx, tiles):
             np.testing.assert_allclose(ev, w[:nr], rtol=0, atol=1e-8)
         b.release(s)
     b.close()
+
+
+@pytest.mark.parametrize("kmax", [28, 44, 64])
+def test_batch_basis_sizes(J, orc, ctx, kmax):
+    """The combine and Gram kernels are instantiated for bases of 16, 32, 48 and 64 vectors,
+    and a launch takes the smallest that holds its largest instance.  A 12-vector instance
+    beside one of up to kmax vectors (then alone once the larger one is done) against dense
+    eigenvalues and the standalone solver with the same settings."""
+    specs = [(3, 3, 7, 0, 7, 0, 1, 3), (3, 2, 8, 0, 8, 0, 1, 2)]   # 1225 and 1568 determinants
+    items = [_build(J, orc, ctx, spec, 700 + i) for i, spec in enumerate(specs)]
+    b = J.Batch(ctx, 16, 16)
+    slots, kws = [], []
+    for (sp, h, w, nr), km in zip(items, [kmax, 12]):
+        kw = dict(nroots=nr, restol=1e-9, krymin=nr + 1, krymax=min(km, sp.ndets),
+                  refdim=min(48, sp.ndets), precision=J.PREC_FP64, guess=J.GUESS_DIAG)
+        kws.append(kw)
+        slots.append(b.add(h, h.params(**kw)))
+    b.run(600)
+    for (sp, h, w, nr), s, kw in zip(items, slots, kws):
+        ev, _, st = b.result(s)
+        assert st["done"] == 1, st
+        alone, _, _ = h.solve(**kw)
+        np.testing.assert_allclose(ev, alone, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(ev, w[:nr], rtol=0, atol=1e-9)
+        b.release(s)
+    b.close()
