@@ -176,7 +176,7 @@ JCI_API int jci_batch_create(jci_ctx *c, int capacity, int max_norbs, jci_batch 
     JCI_CHECK(hipMalloc(&b->dcoef, sizeof(double) * b->coef_cap));
     for (int i = 0; i < JB_STAGES; i++) {
         Stage &g = b->stg[i];
-        JCI_CHECK(hipHostMalloc(&g.hH, sizeof(BHeader), 0));
+        JCI_CHECK(hipHostMalloc(&g.hH, sizeof(BHeader) * (1 + JB_NCLASS), 0));
         JCI_CHECK(hipHostMalloc(&g.hD, sizeof(BDesc) * b->desc_cap, 0));
         JCI_CHECK(hipHostMalloc(&g.hcoef, sizeof(double) * b->coef_cap, 0));
         JCI_CHECK(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming));
@@ -460,6 +460,39 @@ static int upload_coef(jci_batch *b, size_t len)
     return 0;
 }
 
+// Basis classes of the combine and Gram kernels: each class launches the instantiation
+// sized for its largest basis (16, 32, 48 or 64 vectors), so small instances do not run with
+// the registers and LDS of the largest one in the step.
+static int basis_class(int m) { return m <= 16 ? 0 : m <= 32 ? 1 : m <= 48 ? 2 : 3; }
+static const int kClassMax[JB_NCLASS] = {16, 32, 48, JCI_MAXBASIS};
+
+// descriptor order grouped by class (stable inside a class); first[c] / cnt[c]: the class's
+// range in that order
+static void class_order(const std::vector<int> &cls, std::vector<int> &order, int first[JB_NCLASS],
+                        int cnt[JB_NCLASS])
+{
+    order.clear();
+    for (int c = 0; c < JB_NCLASS; c++) {
+        first[c] = (int)order.size();
+        for (int t = 0; t < (int)cls.size(); t++)
+            if (cls[t] == c) order.push_back(t);
+        cnt[c] = (int)order.size() - first[c];
+    }
+}
+
+// one header per class into device slots JB_HCLASS.. (the descriptors go up with upload_desc)
+static int upload_class_headers(jci_batch *b, const int cnt[JB_NCLASS], const long long tot[JB_NCLASS])
+{
+    for (int c = 0; c < JB_NCLASS; c++) {
+        memset(&b->hHs[1 + c], 0, sizeof(BHeader));
+        b->hHs[1 + c].n = cnt[c];
+        b->hHs[1 + c].total = tot[c];
+    }
+    JCI_CHECK(hipMemcpyAsync(b->dH + JB_HCLASS, b->hHs + 1, sizeof(BHeader) * JB_NCLASS,
+                             hipMemcpyHostToDevice, b->c->stream));
+    return 0;
+}
+
 static int download_out(jci_batch *b, size_t len)
 {
     if (len) JCI_CHECK(hipMemcpyAsync(b->hout, b->dout, sizeof(double) * len, hipMemcpyDeviceToHost,
@@ -477,32 +510,41 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
     if (n == 0) { ofs.clear(); return 0; }
     if (n > b->desc_cap) { jci_set_error("batch: %d descriptors above %d", n, b->desc_cap); return -1; }
     RCHECK(stage_next(b));
-    long long blk = 0, pofs = 0;
+    long long pofs = 0, tot[JB_NCLASS] = {0, 0, 0, 0};
     size_t o = 0;
-    int mmax = 1;
+    std::vector<int> cls(n), order;
+    int first[JB_NCLASS], cnt[JB_NCLASS];
     ofs.assign(n, 0);
     for (int t = 0; t < n; t++) {
-        BInst &s = b->in[ids[t]];
-        BDesc &d = b->hD[t];
-        memset(&d, 0, sizeof(d));
         if (m[t] < 1 || w[t] < 1 || m[t] > JCI_MAXBASIS || w[t] > JCI_MAXBASIS) {
             jci_set_error("batch: Gram of %d x %d outside 1..%d", m[t], w[t], JCI_MAXBASIS);
             return -1;
         }
-        mmax = std::max(mmax, std::max(m[t], w[t]));
-        d.N = s.N; d.ld = s.ld; d.A = A[t]; d.B = B[t]; d.m = m[t]; d.k = w[t];
-        d.blk0 = blk; d.nblk = nblk_red(s.N); d.pofs = pofs;
-        d.out = b->dout + o;
+        cls[t] = basis_class(std::max(m[t], w[t]));
         ofs[t] = o;
-        blk += d.nblk;
-        pofs += (long long)d.nblk * m[t] * w[t];
         o += (size_t)m[t] * w[t];
     }
     if (o > b->out_cap) { jci_set_error("batch: scratch too small"); return -1; }
+    class_order(cls, order, first, cnt);
+    for (int q = 0; q < n; q++) {
+        const int t = order[q];
+        BInst &s = b->in[ids[t]];
+        BDesc &d = b->hD[q];
+        memset(&d, 0, sizeof(d));
+        d.N = s.N; d.ld = s.ld; d.A = A[t]; d.B = B[t]; d.m = m[t]; d.k = w[t];
+        d.blk0 = tot[cls[t]]; d.nblk = nblk_red(s.N); d.pofs = pofs;
+        d.out = b->dout + ofs[t];
+        tot[cls[t]] += d.nblk;
+        pofs += (long long)d.nblk * m[t] * w[t];
+    }
     RCHECK(ensure_partial(b, (size_t)pofs));
-    RCHECK(upload_desc(b, n, blk));
+    RCHECK(upload_desc(b, n, tot[0] + tot[1] + tot[2] + tot[3]));
+    RCHECK(upload_class_headers(b, cnt, tot));
     RCHECK(stage_done(b));
-    RCHECK(jb_gram(b->dH, b->dD, n, mmax, b->dpartial, b->c->stream));
+    for (int c = 0; c < JB_NCLASS; c++)
+        if (cnt[c] > 0)
+            RCHECK(jb_gram(b->dH + JB_HCLASS + c, b->dD + first[c], cnt[c], kClassMax[c],
+                           b->dpartial, b->c->stream));
     RCHECK(download_out(b, o));
     return 0;
 }
@@ -517,29 +559,36 @@ static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vec
     int n = (int)ids.size();
     if (n == 0) return 0;
     if (n > b->desc_cap) { jci_set_error("batch: %d descriptors above %d", n, b->desc_cap); return -1; }
-    long long blk = 0;
-    int mmax = 1;
+    long long tot[JB_NCLASS] = {0, 0, 0, 0};
+    std::vector<int> cls(n), order;
+    int first[JB_NCLASS], cnt[JB_NCLASS];
     for (int t = 0; t < n; t++) {
+        if (m[t] > JCI_MAXBASIS || k[t] > m[t]) {
+            jci_set_error("batch: combine of %d vectors into %d (basis limit %d)", m[t], k[t],
+                          JCI_MAXBASIS);
+            return -1;
+        }
+        cls[t] = basis_class(m[t]);
+    }
+    class_order(cls, order, first, cnt);
+    for (int q = 0; q < n; q++) {
+        const int t = order[q];
         BInst &s = b->in[ids[t]];
-        BDesc &d = b->hD[t];
+        BDesc &d = b->hD[q];
         memset(&d, 0, sizeof(d));
         d.N = s.N; d.ld = s.ld; d.A = A[t]; d.O = O[t]; d.m = m[t]; d.k = k[t];
         d.C = b->dcoef + cofs[t]; d.beta = beta;
-        d.blk0 = blk; d.nblk = nblk_rows(s.N);
-        blk += d.nblk;
-        mmax = std::max(mmax, m[t]);
-        if (k[t] > m[t]) {
-            jci_set_error("batch: combine of %d vectors into %d", m[t], k[t]);
-            return -1;
-        }
+        d.blk0 = tot[cls[t]]; d.nblk = nblk_rows(s.N);
+        tot[cls[t]] += d.nblk;
     }
     RCHECK(upload_coef(b, coef_len));
-    RCHECK(upload_desc(b, n, blk));
+    RCHECK(upload_desc(b, n, tot[0] + tot[1] + tot[2] + tot[3]));
+    RCHECK(upload_class_headers(b, cnt, tot));
     RCHECK(stage_done(b));
-    if (jb_combine(b->dH, b->dD, mmax, b->c->stream)) {
-        jci_set_error("batch: combine of %d basis vectors above %d", mmax, JCI_MAXBASIS);
-        return -1;
-    }
+    for (int c = 0; c < JB_NCLASS; c++)
+        if (cnt[c] > 0 && jb_combine(b->dH + JB_HCLASS + c, b->dD + first[c], kClassMax[c],
+                                     b->c->stream))
+            return -1;
     return 0;
 }
 

@@ -12,6 +12,8 @@ struct BHeader {
 };
 
 #define JB_GRID 2048       // persistent work-groups of the vector kernels
+#define JB_NCLASS 4        // basis classes of the combine and Gram launches (16/32/48/64 vectors)
+#define JB_HCLASS 4        // first device header slot of the class launches (slots 4..7)
 #define JB_SGRID 1024      // persistent work-groups of the sigma kernel
 
 // descriptor of one instance for the batched vector kernels (meaning of the fields per
