@@ -146,9 +146,9 @@ def test_batch_pool_rounds_and_buffer_roots(J, orc, ctx, tiles):
 @pytest.mark.parametrize("kmax", [28, 44, 64])
 def test_batch_basis_sizes(J, orc, ctx, kmax):
     """The combine and Gram kernels are instantiated for bases of 16, 32, 48 and 64 vectors,
-    and a launch takes the smallest that holds its largest instance.  A 12-vector instance
-    beside one of up to kmax vectors (then alone once the larger one is done) against dense
-    eigenvalues and the standalone solver with the same settings."""
+    and the instances of a phase are launched in groups by basis size.  A 12-vector instance
+    beside one of up to kmax vectors against dense eigenvalues and the standalone solver
+    with the same settings."""
     specs = [(3, 3, 7, 0, 7, 0, 1, 3), (3, 2, 8, 0, 8, 0, 1, 2)]   # 1225 and 1568 determinants
     items = [_build(J, orc, ctx, spec, 700 + i) for i, spec in enumerate(specs)]
     b = J.Batch(ctx, 16, 16)
