@@ -130,9 +130,13 @@ struct jci_batch {
 static int nblk_rows(long long N) { return (int)((N + 255) / 256); }
 // reduction work-groups per instance: 4096 rows each (enough independent loads in flight to
 // stream the vectors), at most 1024
-static int nblk_red(long long N)
+// blocks of a reduction over N rows: `rows` per block, at most 1024 per instance.  The
+// residual kernel walks its rows 256 at a time and takes 4096.  The Gram kernel walks them
+// 32 at a time (a chain of 128 steps at 4096 rows), so when a launch would not fill the
+// persistent grid it takes 1024 rows per block instead.
+static int nblk_red(long long N, long long rows = 4096)
 {
-    long long b = (N + 4095) / 4096;
+    long long b = (N + rows - 1) / rows;
     return (int)std::max<long long>(1, std::min<long long>(b, 1024));
 }
 
@@ -526,13 +530,16 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
     }
     if (o > b->out_cap) { jci_set_error("batch: scratch too small"); return -1; }
     class_order(cls, order, first, cnt);
+    long long blk4096 = 0;
+    for (int t = 0; t < n; t++) blk4096 += nblk_red(b->in[ids[t]].N);
+    const long long rows = blk4096 < JB_GRID ? 1024 : 4096;
     for (int q = 0; q < n; q++) {
         const int t = order[q];
         BInst &s = b->in[ids[t]];
         BDesc &d = b->hD[q];
         memset(&d, 0, sizeof(d));
         d.N = s.N; d.ld = s.ld; d.A = A[t]; d.B = B[t]; d.m = m[t]; d.k = w[t];
-        d.blk0 = tot[cls[t]]; d.nblk = nblk_red(s.N); d.pofs = pofs;
+        d.blk0 = tot[cls[t]]; d.nblk = nblk_red(s.N, rows); d.pofs = pofs;
         d.out = b->dout + ofs[t];
         tot[cls[t]] += d.nblk;
         pofs += (long long)d.nblk * m[t] * w[t];
