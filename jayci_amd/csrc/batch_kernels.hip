@@ -118,14 +118,17 @@ __global__ __launch_bounds__(256) void k_bgram(const BHeader *__restrict__ H,
     }
 }
 
-// fixed-order sum of the per-block partials of each instance: out = sum_b partial_b
+// fixed-order sum of the per-block partials of each instance: out = sum_b partial_b.
+// blockIdx.x: instance; blockIdx.y: one of BPS_SPLIT blocks sharing its elements, one
+// element per wave at a time (the order of each element's sum does not depend on the split)
+#define BPS_SPLIT 16
 __global__ void k_bpartial_sum(const BHeader *__restrict__ H, const BDesc *__restrict__ D,
                                const double *__restrict__ partial)
 {
     if ((int)blockIdx.x >= H->n) return;
     const BDesc &d = D[blockIdx.x];
     int cnt = d.m * d.k, lane = threadIdx.x & 63;
-    for (int e = threadIdx.x >> 6; e < cnt; e += 4) {
+    for (int e = (int)blockIdx.y * 4 + (threadIdx.x >> 6); e < cnt; e += 4 * (int)gridDim.y) {
         double s = 0.0;
         for (int b = lane; b < d.nblk; b += 64) s += partial[d.pofs + (long long)b * cnt + e];
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -145,7 +148,7 @@ int jb_gram(const BHeader *h, const BDesc *dD, int n_cap, int mmax, double *part
         hipLaunchKernelGGL(k_bgram<48>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
     else
         hipLaunchKernelGGL(k_bgram<JCI_MAXBASIS>, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
-    hipLaunchKernelGGL(k_bpartial_sum, dim3(n_cap), dim3(256), 0, st, h, dD,
+    hipLaunchKernelGGL(k_bpartial_sum, dim3(n_cap, BPS_SPLIT), dim3(256), 0, st, h, dD,
                        (const double *)partial);
     BLAUNCH_CHECK();
     return 0;
