@@ -277,17 +277,20 @@ __global__ void k_bpartial_vec(const BHeader *__restrict__ H, const BDesc *__res
 {
     if ((int)blockIdx.x >= H->n) return;
     const BDesc &d = D[blockIdx.x];
-    for (int r = threadIdx.x; r < d.k; r += blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    // one wave per entry: lanes stride over the blocks, then a fixed shuffle tree
+    for (int r = threadIdx.x >> 6; r < d.k; r += (int)blockDim.x >> 6) {
         double s = 0.0;
-        for (int b = 0; b < d.nblk; b++) s += partial[d.pofs + (long long)b * d.k + r];
-        d.out[r] = s;
+        for (int b = lane; b < d.nblk; b += 64) s += partial[d.pofs + (long long)b * d.k + r];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) d.out[r] = s;
     }
 }
 
 int jb_residual(const BHeader *h, const BDesc *dD, int n_cap, double *partial, hipStream_t st)
 {
     hipLaunchKernelGGL(k_bresidual, dim3(JB_GRID), dim3(256), 0, st, h, dD, partial);
-    hipLaunchKernelGGL(k_bpartial_vec, dim3(n_cap), dim3(64), 0, st, h, dD,
+    hipLaunchKernelGGL(k_bpartial_vec, dim3(n_cap), dim3(256), 0, st, h, dD,
                        (const double *)partial);
     BLAUNCH_CHECK();
     return 0;
