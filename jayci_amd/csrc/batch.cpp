@@ -131,9 +131,9 @@ static int nblk_rows(long long N) { return (int)((N + 255) / 256); }
 // reduction work-groups per instance: 4096 rows each (enough independent loads in flight to
 // stream the vectors), at most 1024
 // blocks of a reduction over N rows: `rows` per block, at most 1024 per instance.  The
-// residual kernel walks its rows 256 at a time and takes 4096.  The Gram kernel walks them
-// 32 at a time (a chain of 128 steps at 4096 rows), so when a launch would not fill the
-// persistent grid it takes 1024 rows per block instead.
+// residual kernel walks its rows 256 at a time and takes 4096; the Gram kernel walks them
+// 32 at a time and takes 1024 (a chain of 32 steps).  The partition depends on N alone, so
+// an instance's sums, and its result, do not depend on the other instances of a launch.
 static int nblk_red(long long N, long long rows = 4096)
 {
     long long b = (N + rows - 1) / rows;
@@ -445,13 +445,15 @@ static int stage_done(jci_batch *b)
     return 0;
 }
 
+// hslot < 0: descriptors only (the launches read per-class headers, upload_class_headers)
 static int upload_desc(jci_batch *b, int n, long long total, int hslot = 0)
 {
     if (n > b->desc_cap) { jci_set_error("batch: %d descriptors above %d", n, b->desc_cap); return -1; }
     b->hHs->n = n;
     b->hHs->total = total;
-    JCI_CHECK(hipMemcpyAsync(b->dH + hslot, b->hHs, sizeof(BHeader), hipMemcpyHostToDevice,
-                             b->c->stream));
+    if (hslot >= 0)
+        JCI_CHECK(hipMemcpyAsync(b->dH + hslot, b->hHs, sizeof(BHeader), hipMemcpyHostToDevice,
+                                 b->c->stream));
     if (n > 0)
         JCI_CHECK(hipMemcpyAsync(b->dD, b->hD, sizeof(BDesc) * n, hipMemcpyHostToDevice, b->c->stream));
     return 0;
@@ -530,22 +532,19 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
     }
     if (o > b->out_cap) { jci_set_error("batch: scratch too small"); return -1; }
     class_order(cls, order, first, cnt);
-    long long blk4096 = 0;
-    for (int t = 0; t < n; t++) blk4096 += nblk_red(b->in[ids[t]].N);
-    const long long rows = blk4096 < JB_GRID ? 1024 : 4096;
     for (int q = 0; q < n; q++) {
         const int t = order[q];
         BInst &s = b->in[ids[t]];
         BDesc &d = b->hD[q];
         memset(&d, 0, sizeof(d));
         d.N = s.N; d.ld = s.ld; d.A = A[t]; d.B = B[t]; d.m = m[t]; d.k = w[t];
-        d.blk0 = tot[cls[t]]; d.nblk = nblk_red(s.N, rows); d.pofs = pofs;
+        d.blk0 = tot[cls[t]]; d.nblk = nblk_red(s.N, 1024); d.pofs = pofs;
         d.out = b->dout + ofs[t];
         tot[cls[t]] += d.nblk;
         pofs += (long long)d.nblk * m[t] * w[t];
     }
     RCHECK(ensure_partial(b, (size_t)pofs));
-    RCHECK(upload_desc(b, n, tot[0] + tot[1] + tot[2] + tot[3]));
+    RCHECK(upload_desc(b, n, tot[0] + tot[1] + tot[2] + tot[3], -1));
     RCHECK(upload_class_headers(b, cnt, tot));
     RCHECK(stage_done(b));
     for (int c = 0; c < JB_NCLASS; c++)
@@ -589,7 +588,7 @@ static int run_combine(jci_batch *b, const std::vector<int> &ids, const std::vec
         tot[cls[t]] += d.nblk;
     }
     RCHECK(upload_coef(b, coef_len));
-    RCHECK(upload_desc(b, n, tot[0] + tot[1] + tot[2] + tot[3]));
+    RCHECK(upload_desc(b, n, tot[0] + tot[1] + tot[2] + tot[3], -1));
     RCHECK(upload_class_headers(b, cnt, tot));
     RCHECK(stage_done(b));
     for (int c = 0; c < JB_NCLASS; c++)

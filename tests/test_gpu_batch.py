@@ -167,3 +167,30 @@ def test_batch_basis_sizes(J, orc, ctx, kmax):
         np.testing.assert_allclose(ev, w[:nr], rtol=0, atol=1e-9)
         b.release(s)
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["PREC_FP64", "PREC_MIXED"])
+def test_batch_result_independent_of_mix(J, orc, ctx, precision):
+    """An instance's eigenvalues and iteration count are bit-identical whether it runs alone
+    or beside other instances (of other basis sizes, so other kernel instantiations and
+    launch groupings)."""
+    prec = getattr(J, precision)
+    target = (3, 3, 8, 0, 8, 0, 1, 2)
+    others = [(4, 4, 9, 0, 9, 0, 1, 4), (2, 2, 6, 0, 6, 0, 1, 1), (3, 3, 7, 1, 3, 2, 0, 1)]
+    runs = []
+    for mix in ([], others):
+        items = [_build(J, orc, ctx, target, 900)] + [_build(J, orc, ctx, s, 901 + i)
+                                                       for i, s in enumerate(mix)]
+        b = J.Batch(ctx, 16, 16)
+        slots = []
+        for (sp, h, w, nr), km in zip(items, [40, 44, 12, 16]):
+            slots.append(b.add(h, h.params(nroots=nr, restol=1e-8, krymin=nr + 1,
+                                           krymax=min(km, sp.ndets), refdim=min(48, sp.ndets),
+                                           precision=prec, guess=J.GUESS_DIAG)))
+        b.run(600)
+        ev, _, st = b.result(slots[0])
+        assert st["done"] == 1, st
+        runs.append((ev.copy(), st["iterations"], st["nsigma64"], st["nsigma32"]))
+        b.close()
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1:] == runs[1][1:]
