@@ -130,14 +130,16 @@ struct jci_batch {
 static int nblk_rows(long long N) { return (int)((N + 255) / 256); }
 // reduction work-groups per instance: 4096 rows each (enough independent loads in flight to
 // stream the vectors), at most 1024
-// blocks of a reduction over N rows: `rows` per block, at most 1024 per instance.  The
-// residual kernel walks its rows 256 at a time and takes 4096; the Gram kernel walks them
-// 32 at a time and takes 1024 (a chain of 32 steps).  The partition depends on N alone, so
-// an instance's sums, and its result, do not depend on the other instances of a launch.
-static int nblk_red(long long N, long long rows = 4096)
+// blocks of a reduction over N rows: `rows` per block, at most `cap` per instance.  The
+// Gram kernel walks its rows 32 at a time and takes 1024 (a chain of 32 steps), at most 1024
+// blocks (each leaves an m x w partial).  The residual kernel walks them 256 at a time and
+// also takes 1024, at most 8192 blocks (k partials each), so that a 4M-row instance spreads
+// over more blocks than the GPU holds at once.  The partition depends on N alone, so an
+// instance's sums, and its result, do not depend on the other instances of a launch.
+static int nblk_red(long long N, long long rows, long long cap)
 {
     long long b = (N + rows - 1) / rows;
-    return (int)std::max<long long>(1, std::min<long long>(b, 1024));
+    return (int)std::max<long long>(1, std::min<long long>(b, cap));
 }
 
 // grow the partial-sum buffer (reductions) to hold `need` doubles
@@ -538,7 +540,7 @@ static int run_gram(jci_batch *b, const std::vector<int> &ids, std::vector<doubl
         BDesc &d = b->hD[q];
         memset(&d, 0, sizeof(d));
         d.N = s.N; d.ld = s.ld; d.A = A[t]; d.B = B[t]; d.m = m[t]; d.k = w[t];
-        d.blk0 = tot[cls[t]]; d.nblk = nblk_red(s.N, 1024); d.pofs = pofs;
+        d.blk0 = tot[cls[t]]; d.nblk = nblk_red(s.N, 1024, 1024); d.pofs = pofs;
         d.out = b->dout + ofs[t];
         tot[cls[t]] += d.nblk;
         pofs += (long long)d.nblk * m[t] * w[t];
@@ -921,7 +923,7 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             for (int r = 0; r < k; r++) b->hcoef[co + (size_t)m * k + r] = s.theta[r];
             d.N = s.N; d.ld = s.ld; d.A = s.V; d.B = s.S; d.O = s.T; d.m = m; d.k = k;
             d.C = b->dcoef + co; d.th = b->dcoef + co + (size_t)m * k; d.hd = s.fa.hd;
-            d.out = b->dout + o; d.blk0 = blk; d.nblk = nblk_red(s.N); d.pofs = pofs;
+            d.out = b->dout + o; d.blk0 = blk; d.nblk = nblk_red(s.N, 1024, 8192); d.pofs = pofs;
             ofs[t] = o;
             co += (size_t)m * k + k;
             o += k;
