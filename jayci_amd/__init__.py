@@ -400,6 +400,9 @@ class Hamiltonian:
         g = None
         if guess == GUESS_USER:
             g = np.ascontiguousarray(guess_vectors, dtype=np.float64)
+            if g.ndim != 2 or g.shape[1] != N or g.shape[0] < p.krymin:
+                raise ValueError("guess_vectors must hold krymin=%d vectors of %d, got shape %s"
+                                 % (p.krymin, N, g.shape))
         st = SolveStats()
         rc = _L.jci_solve(self._h, ctypes.byref(p), _p(g), _p(ev), _p(vec), ctypes.byref(st))
         if check and rc != 0:

@@ -159,7 +159,7 @@ struct ListOp {
     std::vector<int> up_col;
     // matrix elements for one set of integrals
     const double *key_m1 = nullptr, *key_m2 = nullptr;
-    double key_sum = 0.0;
+    uint64_t key_sum = 0;
     double *d_m1 = nullptr, *d_m2 = nullptr, *d_hd = nullptr, *d_val = nullptr;
     long long *d_rowptr = nullptr;
     int *d_col = nullptr;
@@ -242,21 +242,30 @@ static void op_pattern_from_hmap(ListOp *op, const struct rowmap *hmap)
     op->up_ptr[n] = (long long)op->up_col.size();
 }
 
-static double int_checksum(const double *m1, const double *m2, int n)
+// 64-bit hash of every word of the integral arrays (bit patterns, so any change to any
+// integral changes the key): the cached matrix elements are reused only for the same values
+static uint64_t int_hash(const double *m1, const double *m2, int n)
 {
-    size_t a = m1_len(n), b = m2_len(n);
-    double s = 0.0;
-    for (size_t i = 0; i < a; i++) s += m1[i] * (double)(i % 97 + 1);
-    size_t step = b > 4096 ? b / 4096 : 1;
-    for (size_t i = 0; i < b; i += step) s += m2[i] * (double)(i % 89 + 1);
-    return s;
+    uint64_t h = 0x9e3779b97f4a7c15ULL;
+    auto mix = [&](const double *a, size_t len) {
+        for (size_t i = 0; i < len; i++) {
+            uint64_t w;
+            memcpy(&w, a + i, sizeof(w));
+            h ^= w + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+            h *= 0xff51afd7ed558ccdULL;
+            h ^= h >> 33;
+        }
+    };
+    mix(m1, m1_len(n));
+    mix(m2, m2_len(n));
+    return h;
 }
 
 // matrix elements of the full pattern (both triangles, diagonal separate) for m1/m2
 static int op_values(ListOp *op, const double *m1, const double *m2)
 {
     int n_orb = op->norbs;
-    double sum = int_checksum(m1, m2, n_orb);
+    const uint64_t sum = int_hash(m1, m2, n_orb);
     if (op->d_val && op->key_m1 == m1 && op->key_m2 == m2 && op->key_sum == sum) return 0;
     jci_ctx *c = op->c;
     long long n = op->n;
@@ -513,7 +522,13 @@ JCI_API int dvdalg(struct det *dlist, int ndets, double *moints1, double *moints
         std::vector<double *> rows(krymin);
         for (int r = 0; r < krymin; r++) rows[r] = store.data() + (size_t)r * ndets;
         rc = read_civectors(aelec, belec, norbs, rows.data(), ndets, krymin, ev.data());
-        if (rc) return rc;
+        if (rc) {
+            // source/initguess_roldv.c:30-41: 1 (no file) and 2 (header) pass through, any
+            // other reader code becomes 100
+            if (rc != 1 && rc != 2) rc = 100;
+            report(rc, "dvdalg");
+            return rc;
+        }
         guess.swap(store);
         p.guess = JCI_GUESS_USER;
     } else {

@@ -1335,17 +1335,48 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     // initial space
     if (p->guess == JCI_GUESS_USER) {
         if (!guess) { jci_set_error("solve: user guess requested without vectors"); rc = -1; goto done; }
-        for (int i = 0; i < krymin; i++) {
+        // the caller supplies p->krymin vectors; the effective krymin can be larger (tracked
+        // buffer pairs), and the vectors beyond the caller's come from the block guess
+        const int nsup = std::max(0, std::min(p->krymin, krymin));
+        for (int i = 0; i < nsup; i++) {
             if (hipMemcpyAsync(w.T + (size_t)(i % JCI_MAXK) * w.ld, guess + (size_t)i * N,
                                sizeof(double) * N, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
                 rc = -2; goto done;
             }
-            if (i % JCI_MAXK == JCI_MAXK - 1 || i == krymin - 1) {
+            if (i % JCI_MAXK == JCI_MAXK - 1 || i == nsup - 1) {
                 int k = i % JCI_MAXK + 1, kept;
                 if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;
                 if (hipMemcpyAsync(w.V + (size_t)m * w.ld, w.T, vb * kept, hipMemcpyDeviceToDevice,
                                    c->stream) != hipSuccess) { rc = -2; goto done; }
                 m += kept;
+            }
+        }
+        if (m < krymin) {
+            // block-guess vectors (staged in S, which holds no sigma yet) fill the rest,
+            // orthonormalised against the caller's
+            std::vector<long long> idx;
+            std::vector<double> U;
+            if ((rc = guess_compute(op, c, p, krymin, idx, U))) goto done;
+            void *scr = nullptr;
+            if (hipMalloc(&scr, (sizeof(long long) + sizeof(double) * krymin) * idx.size()) != hipSuccess) {
+                jci_set_error("solve: out of device memory for the guess block");
+                rc = -3; goto done;
+            }
+            rc = guess_apply(idx, U, krymin, w.S, w.ld, scr, c->stream);
+            if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = -2;
+            hipFree(scr);
+            if (rc) goto done;
+            for (int i = 0; i < krymin && m < krymin;) {
+                const int k = std::min(JCI_MAXK, krymin - i);
+                int kept;
+                if (hipMemcpyAsync(w.T, w.S + (size_t)i * w.ld, vb * k, hipMemcpyDeviceToDevice,
+                                   c->stream) != hipSuccess) { rc = -2; goto done; }
+                if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;
+                kept = std::min(kept, krymin - m);
+                if (hipMemcpyAsync(w.V + (size_t)m * w.ld, w.T, vb * kept, hipMemcpyDeviceToDevice,
+                                   c->stream) != hipSuccess) { rc = -2; goto done; }
+                m += kept;
+                i += k;
             }
         }
     } else if (p->guess == JCI_GUESS_UNIT) {
