@@ -226,6 +226,17 @@ int jci_ham_rows(jci_ham *h, const long long *rows, int nrows, long long cap, lo
 /* y = H x for nvec vectors stored determinant-major ([N][nvec]); _dev: device memory */
 int jci_sigma_dev(jci_ham *h, int method, int nvec, const double *x_dev, double *y_dev);
 int jci_sigma(jci_ham *h, int method, int nvec, const double *x, double *y);
+/* jci_sigma_dev without any device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, serving loops that forbid hipMalloc).
+ * Every buffer beyond x and y (the FP32 copy of x, the dense-string intermediate and its
+ * descriptors, vector staging) comes from one library-owned device array of
+ * jci_sigma_noalloc_capacity() bytes (128 MiB) per GPU, kept until the process ends; a call
+ * needing more returns -1 with the sizes in jci_last_error() (the dense-string sigma then
+ * runs its vectors in as many passes as the array allows, one vector of up to about
+ * 128 MiB / (8 La) determinants in FP64).  All calls on one GPU share the array, so calls
+ * that overlap on one GPU race; results equal jci_sigma_dev's bit for bit. */
+int jci_sigma_dev_noalloc(jci_ham *h, int method, int nvec, const double *x_dev, double *y_dev);
+long long jci_sigma_noalloc_capacity(void);
 
 typedef struct {
     int maxiter, krymin, krymax, nroots, refdim;

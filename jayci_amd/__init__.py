@@ -77,6 +77,8 @@ _sig("jci_ham_npairs", _ll, [_P])
 _sig("jci_ham_hdiag", _i, [_P, _P])
 _sig("jci_ham_lowest_diag", _i, [_P, _i, _P])
 _sig("jci_ham_rows", _i, [_P, _P, _i, _ll, _P, _P, _P, _P, _P])
+_sig("jci_sigma_dev_noalloc", _i, [_P, _i, _i, _P, _P])
+_sig("jci_sigma_noalloc_capacity", _ll, [])
 _sig("jci_sigma_dev", _i, [_P, _i, _i, _P, _P])
 _sig("jci_sigma", _i, [_P, _i, _i, _P, _P])
 _sig("jci_solve_defaults", None, [ctypes.POINTER(SolveParams)])

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjayci_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["kernels.hip", "sigma_fast.hip", "sigma_gemm.hip", "batch_kernels.hip"]
+HIP_SOURCES = ["kernels.hip", "sigma_fast.hip", "sigma_gemm.hip", "batch_kernels.hip",
+               "noalloc.hip"]
 CXX_SOURCES = ["space.cpp", "jci.cpp", "capi.cpp", "batch.cpp", "symeig.cpp", "sifs.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden", "-Wno-unused-result", "-Wno-unused-value", "-I" + CSRC,

@@ -85,6 +85,8 @@ JCI_API int jci_ctx_create(int device, jci_ctx **out)
     JCI_CHECK(hipMalloc(&c->d_binom, sizeof(unsigned long long) * 65 * 65));
     JCI_CHECK(hipMemcpy(c->d_binom, tab.data(), sizeof(unsigned long long) * 65 * 65,
                         hipMemcpyHostToDevice));
+    int rc = jci_noalloc_setup(device);
+    if (rc) return rc;
     *out = c;
     return 0;
 }
@@ -645,8 +647,25 @@ static int resolve_method(const jci_ham *h, int method, int want32)
     return h->have_fast ? JCI_SIGMA_FAST64 : JCI_SIGMA_DIRECT64;
 }
 
-static int ensure_xf(jci_ham *h, long long n)
+// a caller-provided device block carved into the buffers one sigma call needs (the
+// allocation-free entry point); every carve is 256-byte aligned
+int jci_scratch_carve(SigmaScratch *s, size_t bytes, void **out, const char *what)
 {
+    const size_t at = (s->used + 255) / 256 * 256;
+    if (at > s->size || bytes > s->size - at) {
+        jci_set_error("sigma: %s needs %zu bytes at offset %zu of the %zu-byte scratch array "
+                      "of the allocation-free entry point", what, bytes, at, s->size);
+        return -1;
+    }
+    *out = s->base + at;
+    s->used = at + bytes;
+    return 0;
+}
+
+static int ensure_xf(jci_ham *h, long long n, SigmaScratch *s, float **xf)
+{
+    if (s) return jci_scratch_carve(s, sizeof(float) * (size_t)n, (void **)xf, "FP32 staging");
+    *xf = h->d_xf;
     if (h->xf_len >= n) return 0;
     hipFree(h->d_xf);
     h->d_xf = nullptr;
@@ -656,6 +675,7 @@ static int ensure_xf(jci_ham *h, long long n)
         return -3;
     }
     h->xf_len = n;
+    *xf = h->d_xf;
     return 0;
 }
 
@@ -675,11 +695,69 @@ static int grow(void **p, size_t *have, size_t want, const char *what)
 
 // dense-string sigma of K vector-major vectors: y + k*ldy = H (x + k*ldx)
 static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, double *y,
-                  long long ldy)
+                  long long ldy, SigmaScratch *s)
 {
     jci_ctx *c = h->sp->ctx;
     const long long N = h->sp->h.ndets;
     const size_t esz = prec == 64 ? 8 : 4;
+    if (s) {
+        // allocation-free: descriptors, the intermediate and the FP32 copies in the scratch
+        // block, the vectors in passes of as many as the block holds
+        GfTask *d_t;
+        GfHeader *d_h;
+        RCHECK(jci_scratch_carve(s, sizeof(GfTask), (void **)&d_t, "the task descriptor"));
+        RCHECK(jci_scratch_carve(s, sizeof(GfHeader), (void **)&d_h, "the task header"));
+        // vectors per pass: the most whose intermediate and FP32 copies fit what is left
+        const size_t at = (s->used + 255) / 256 * 256;
+        const size_t left = at < s->size ? s->size - at : 0;
+        auto pass_bytes = [&](int k) {
+            return (size_t)h->gf.La * jk_gf_lde((long long)k * N) * esz + 256 +
+                   (prec == 32 ? sizeof(float) * (size_t)k * N + 256 : 0);
+        };
+        int Kc = K;
+        while (Kc > 0 && pass_bytes(Kc) > left) Kc--;
+        if (Kc < 1) {
+            jci_set_error("sigma: the dense-string intermediate of %lld determinants needs %zu "
+                          "bytes, more than the %zu-byte scratch array of the allocation-free "
+                          "entry point holds", N, pass_bytes(1), s->size);
+            return -1;
+        }
+        const size_t mark = s->used;
+        for (int b = 0; b < K; b += Kc) {
+            const int k = std::min(Kc, K - b);
+            s->used = mark;
+            void *E;
+            float *x32 = nullptr;
+            RCHECK(jci_scratch_carve(s, (size_t)h->gf.La * jk_gf_lde((long long)k * N) * esz, &E,
+                                     "the dense-string intermediate"));
+            if (prec == 32)
+                RCHECK(jci_scratch_carve(s, sizeof(float) * (size_t)k * N, (void **)&x32,
+                                         "FP32 staging"));
+            GfTask t;
+            GfHeader hh;
+            memset(&t, 0, sizeof(t));
+            memset(&hh, 0, sizeof(hh));
+            t.g = h->gf;
+            t.K = k;
+            t.x64 = x + (size_t)b * ldx;
+            t.ldx64 = ldx;
+            t.x32 = x32;
+            t.y = y + (size_t)b * ldy;
+            t.ldy = ldy;
+            t.E = E;
+            t.lde = jk_gf_lde((long long)k * N);
+            hh.n = 1;
+            const int bn = prec == 64 ? h->gf_bn64 : h->gf_bn32;
+            jk_gf_units(prec, bn, t, hh.total);
+            // the descriptors come from this stack frame: wait for the copies before the
+            // frame is reused (earlier passes' kernels are ordered after their own copies)
+            JCI_CHECK(hipMemcpyAsync(d_t, &t, sizeof(t), hipMemcpyHostToDevice, c->stream));
+            JCI_CHECK(hipMemcpyAsync(d_h, &hh, sizeof(hh), hipMemcpyHostToDevice, c->stream));
+            JCI_CHECK(hipStreamSynchronize(c->stream));
+            RCHECK(jk_gf_group(prec, bn, d_h, d_t, &hh, c->stream));
+        }
+        return 0;
+    }
     if (!h->d_gft) {
         JCI_CHECK(hipMalloc(&h->d_gft, sizeof(GfTask) * JCI_MAXK));
         JCI_CHECK(hipMalloc(&h->d_gfh, sizeof(GfHeader) * JCI_MAXK));
@@ -728,12 +806,15 @@ static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, d
 
 static bool is_gf(int method) { return method == JCI_SIGMA_GEMM64 || method == JCI_SIGMA_GEMM32; }
 
-// y = H x on device, x and y determinant-major [N][nvec]
-static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *y)
+// y = H x on device, x and y determinant-major [N][nvec]; s (may be NULL) supplies every
+// device buffer the call needs beyond x and y (the allocation-free entry point)
+static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *y,
+                     SigmaScratch *s = nullptr)
 {
-    jci_space *s = h->sp;
-    jci_ctx *c = s->ctx;
-    long long N = s->h.ndets;
+    jci_space *sp = h->sp;
+    jci_ctx *c = sp->ctx;
+    long long N = sp->h.ndets;
+    float *xf = nullptr;
     if (nvec < 1 || nvec > JCI_MAXK) { jci_set_error("sigma: nvec must be 1..%d", JCI_MAXK); return -1; }
     method = resolve_method(h, method, 0);
     switch (method) {
@@ -742,9 +823,9 @@ static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *
         return jk_spmv(64, nvec, N, h->d_rowptr, h->d_col, h->d_val64, h->d_hd, x, y, c->stream);
     case JCI_SIGMA_PAIRS32:
         if (!h->d_val32 || h->upper) { jci_set_error("sigma: no full FP32 pair list"); return -1; }
-        RCHECK(ensure_xf(h, N * nvec));
-        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
-        return jk_spmv(32, nvec, N, h->d_rowptr, h->d_col, h->d_val32, h->d_hd, h->d_xf, y, c->stream);
+        RCHECK(ensure_xf(h, N * nvec, s, &xf));
+        RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
+        return jk_spmv(32, nvec, N, h->d_rowptr, h->d_col, h->d_val32, h->d_hd, xf, y, c->stream);
     case JCI_SIGMA_PAIRS_SYM64:
         if (!h->d_val64 || !h->upper) { jci_set_error("sigma: no upper FP64 pair list"); return -1; }
         if (nvec == 1)
@@ -752,39 +833,46 @@ static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *
         jci_set_error("sigma: symmetric pair list takes one vector per call");
         return -1;
     case JCI_SIGMA_DIRECT64:
-        return jk_sigma_generic(64, nvec, s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom,
+        return jk_sigma_generic(64, nvec, sp->d_desc, N, sp->d_mask[0], sp->d_mask[1], c->d_binom,
                                 h->d_m1, h->d_m2, h->d_hd, x, y, c->stream);
     case JCI_SIGMA_DIRECT32:
-        RCHECK(ensure_xf(h, N * nvec));
-        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
-        return jk_sigma_generic(32, nvec, s->d_desc, N, s->d_mask[0], s->d_mask[1], c->d_binom,
-                                h->d_m1, h->d_m2, h->d_hd, h->d_xf, y, c->stream);
+        RCHECK(ensure_xf(h, N * nvec, s, &xf));
+        RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
+        return jk_sigma_generic(32, nvec, sp->d_desc, N, sp->d_mask[0], sp->d_mask[1], c->d_binom,
+                                h->d_m1, h->d_m2, h->d_hd, xf, y, c->stream);
     case JCI_SIGMA_FAST64:
         if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
         return jk_sigma_fast(64, nvec, h->fast[h->fast_w], x, y, c->stream);
     case JCI_SIGMA_FAST32:
         if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
-        RCHECK(ensure_xf(h, N * nvec));
-        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
-        return jk_sigma_fast(32, nvec, h->fast[h->fast_w], h->d_xf, y, c->stream);
+        RCHECK(ensure_xf(h, N * nvec, s, &xf));
+        RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
+        return jk_sigma_fast(32, nvec, h->fast[h->fast_w], xf, y, c->stream);
     case JCI_SIGMA_FCI64:
         if (!h->have_fci) { jci_set_error("sigma: single-block kernel not applicable"); return -1; }
         return jk_sigma_fci(64, nvec, h->fci, x, y, c->stream);
     case JCI_SIGMA_FCI32:
         if (!h->have_fci) { jci_set_error("sigma: single-block kernel not applicable"); return -1; }
-        RCHECK(ensure_xf(h, N * nvec));
-        RCHECK(jk_to_f32(x, h->d_xf, N * nvec, c->stream));
-        return jk_sigma_fci(32, nvec, h->fci, h->d_xf, y, c->stream);
+        RCHECK(ensure_xf(h, N * nvec, s, &xf));
+        RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
+        return jk_sigma_fci(32, nvec, h->fci, xf, y, c->stream);
     case JCI_SIGMA_GEMM64:
     case JCI_SIGMA_GEMM32: {
         if (!h->have_gf) { jci_set_error("sigma: dense-string sigma not applicable"); return -1; }
         const int prec = method == JCI_SIGMA_GEMM64 ? 64 : 32;
-        if (nvec == 1) return gf_run(h, prec, 1, x, N, y, N);
-        RCHECK(grow((void **)&h->d_gfT, &h->gfT_len, sizeof(double) * 2 * (size_t)nvec * N,
-                    "vector staging"));
-        double *t1 = h->d_gfT, *t2 = h->d_gfT + (size_t)nvec * N;
+        if (nvec == 1) return gf_run(h, prec, 1, x, N, y, N, s);
+        double *t1;
+        if (s) {
+            RCHECK(jci_scratch_carve(s, sizeof(double) * 2 * (size_t)nvec * N, (void **)&t1,
+                                     "vector staging"));
+        } else {
+            RCHECK(grow((void **)&h->d_gfT, &h->gfT_len, sizeof(double) * 2 * (size_t)nvec * N,
+                        "vector staging"));
+            t1 = h->d_gfT;
+        }
+        double *t2 = t1 + (size_t)nvec * N;
         RCHECK(jk_unpack(nvec, N, x, t1, N, c->stream));
-        RCHECK(gf_run(h, prec, nvec, t1, N, t2, N));
+        RCHECK(gf_run(h, prec, nvec, t1, N, t2, N, s));
         return jk_pack(nvec, N, t2, N, y, c->stream);
     }
     default:
@@ -797,6 +885,15 @@ JCI_API int jci_sigma_dev(jci_ham *h, int method, int nvec, const double *x_dev,
 {
     JCI_CHECK(hipSetDevice(h->sp->ctx->device));
     return sigma_dev(h, method, nvec, x_dev, y_dev);
+}
+
+int jci_ham_device(const jci_ham *h) { return h->sp->ctx->device; }
+
+int jci_sigma_dev_scratch(jci_ham *h, int method, int nvec, const double *x_dev, double *y_dev,
+                          SigmaScratch *s)
+{
+    JCI_CHECK(hipSetDevice(h->sp->ctx->device));
+    return sigma_dev(h, method, nvec, x_dev, y_dev, s);
 }
 
 JCI_API int jci_sigma(jci_ham *h, int method, int nvec, const double *x, double *y)
@@ -1592,7 +1689,7 @@ static int ham_sigma_cols(void *self, int method, int k, const double *V, long l
     method = resolve_method(h, method, 0);
     if (!is_gf(method)) return 1;
     if (!h->have_gf) { jci_set_error("sigma: dense-string sigma not applicable"); return -1; }
-    return gf_run(h, method == JCI_SIGMA_GEMM64 ? 64 : 32, k, V, ldv, S, lds);
+    return gf_run(h, method == JCI_SIGMA_GEMM64 ? 64 : 32, k, V, ldv, S, lds, nullptr);
 }
 
 static int ham_masks(void *self, long long n, const long long *d_idx, uint64_t *oa, uint64_t *ob)

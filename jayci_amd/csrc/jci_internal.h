@@ -61,6 +61,23 @@ struct jci_ctx {
     double *h_small;            // pinned mirror of d_small
 };
 
+// ---- caller-provided sigma buffers (jci.cpp, the allocation-free entry point) ------------
+// A device block of `size` bytes at `base`; jci_scratch_carve hands out 256-byte aligned
+// pieces from `used` on and fails with -1 past the end.
+struct jci_ham;
+struct SigmaScratch {
+    unsigned char *base;
+    size_t size, used;
+};
+int jci_scratch_carve(SigmaScratch *s, size_t bytes, void **out, const char *what);
+// jci_sigma_dev taking every buffer beyond x and y from s (no device allocation)
+int jci_sigma_dev_scratch(jci_ham *h, int method, int nvec, const double *x_dev, double *y_dev,
+                          SigmaScratch *s);
+int jci_ham_device(const jci_ham *h);
+// looks up this device's copy of the allocation-free entry point's array (noalloc.hip);
+// called once per device with the device current
+int jci_noalloc_setup(int device);
+
 // ---- Davidson over an abstract operator (jci.cpp) -------------------------------------
 struct SolveOp {
     void *self;
