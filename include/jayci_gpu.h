@@ -223,6 +223,18 @@ int jci_ham_lowest_diag(jci_ham *h, int nb, long long *idx);
  * row t's entries at [t*cap, t*cap + counts[t]) */
 int jci_ham_rows(jci_ham *h, const long long *rows, int nrows, long long cap, long long *cols,
                  double *vals, int *phase, int *deg, long long *counts);
+/* excitation-table entries of one string (Hamiltonians built with JCI_DIRECT_TABLES), the
+ * per-string singles (kind 1) or doubles (kind 2) to strings of the space, as the matrix-free
+ * sigma reads them: targets[] the target string's index in its spin's string list, info[]
+ * singles i | f<<6 | negative<<12 | target group<<13 | pair tri(i,f)<<18 (tri(a,b) = a(a+1)/2 + b
+ * for a >= b), doubles i0 | i1<<6 | f0<<12 | f1<<18 |
+ * negative<<24 | target group<<25 (0-based orbitals), vals[] the one-spin element (singles:
+ * unsigned h_if + sum_{k in string, k != i} [(kk|if) - (ki|kf)]; doubles: signed).  No
+ * counterpart in the reference (its strings carry yij/cij tables instead,
+ * mpi_source/citruncate.c:258-342); for checking the tables.  Returns the entry count (up
+ * to cap written) or < 0. */
+long long jci_ham_string_excitations(jci_ham *h, int spin, long long string, int kind,
+                                     long long cap, long long *targets, int *info, double *vals);
 /* y = H x for nvec vectors stored determinant-major ([N][nvec]); _dev: device memory */
 int jci_sigma_dev(jci_ham *h, int method, int nvec, const double *x_dev, double *y_dev);
 int jci_sigma(jci_ham *h, int method, int nvec, const double *x, double *y);
@@ -317,6 +329,11 @@ int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32, d
 /* times[] above has JCI_SIGMA_GEMM32 + 1 entries */
 /* beta strings per tile of the matrix-free sigma (0 without excitation tables) */
 int jci_ham_fast_width(const jci_ham *h);
+/* tile shape of the matrix-free table sigma (the autotuner sets it): *width beta strings per
+ * work-group (64, 128 or 256) and *chunk alpha singles staged together (1, 2 or 4; 0 = the
+ * most that fit 64 KB of LDS, at most 4); positive values set, -1 / 0 keep (width) or
+ * choose automatically (chunk 0); both are returned */
+int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk);
 /* column-tile widths of the dense-string sigma's opposite-spin product (FP64: 16 or 32,
  * FP32: 32 or 64; the autotuner sets them): a positive *bn sets, then both are returned */
 int jci_ham_gemm_tiles(jci_ham *h, int *bn64, int *bn32);

@@ -77,6 +77,7 @@ _sig("jci_ham_npairs", _ll, [_P])
 _sig("jci_ham_hdiag", _i, [_P, _P])
 _sig("jci_ham_lowest_diag", _i, [_P, _i, _P])
 _sig("jci_ham_rows", _i, [_P, _P, _i, _ll, _P, _P, _P, _P, _P])
+_sig("jci_ham_string_excitations", _ll, [_P, _i, _ll, _i, _ll, _P, _P, _P])
 _sig("jci_sigma_dev_noalloc", _i, [_P, _i, _i, _P, _P])
 _sig("jci_sigma_noalloc_capacity", _ll, [])
 _sig("jci_sigma_dev", _i, [_P, _i, _i, _P, _P])
@@ -90,6 +91,7 @@ _sig("jci_bench_sigma", _i, [_P, _i, _i, _i, ctypes.POINTER(_d)])
 _sig("jci_bench_copy", _i, [_P, _ll, _i, ctypes.POINTER(_d)])
 _sig("jci_ham_autotune", _i, [_P, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _P])
 _sig("jci_ham_fast_width", _i, [_P])
+_sig("jci_ham_fast_tiles", _i, [_P, ctypes.POINTER(_i), ctypes.POINTER(_i)])
 _sig("jci_batch_create", _i, [_P, _i, _i, ctypes.POINTER(_P)])
 _sig("jci_batch_destroy", None, [_P])
 _sig("jci_batch_set_graphs", _i, [_P, _i])
@@ -345,6 +347,18 @@ class Hamiltonian:
             out.append((cols[sl][o], vals[sl][o], ph[sl][o], dg[sl][o]))
         return out
 
+    def string_excitations(self, spin, string, kind):
+        """Excitation-table entries of one string: (targets, info, values) of its singles
+        (kind 1) or doubles (kind 2); see jci_ham_string_excitations."""
+        n = _L.jci_ham_string_excitations(self._h, spin, string, kind, 0, None, None, None)
+        if n < 0:
+            raise JciError("jci_ham_string_excitations failed: %s" % last_error())
+        t = np.zeros(n, dtype=np.int64)
+        info = np.zeros(n, dtype=np.int32)
+        v = np.zeros(n)
+        _L.jci_ham_string_excitations(self._h, spin, string, kind, n, _p(t), _p(info), _p(v))
+        return t, info, v
+
     def bench_sigma(self, method=SIGMA_AUTO, nvec=1, reps=20):
         """Mean ms per sigma launch (HIP events on the context stream)."""
         ms = _d()
@@ -359,6 +373,12 @@ class Hamiltonian:
                "jci_ham_autotune")
         return b64.value, b32.value, {m: t[m] for m in range(NMETHODS) if t[m] > 0}, \
             _L.jci_ham_fast_width(self._h)
+
+    def fast_tiles(self, width=0, chunk=-1):
+        """Tile shape of the table sigma (jci_ham_fast_tiles): returns (width, chunk)."""
+        w, c = _i(width), _i(chunk)
+        _check(_L.jci_ham_fast_tiles(self._h, ctypes.byref(w), ctypes.byref(c)), "jci_ham_fast_tiles")
+        return w.value, c.value
 
     def gemm_tiles(self, bn64=0, bn32=0):
         """E tile widths of the dense-string sigma; positive arguments set them."""

@@ -502,6 +502,7 @@ static int build_fast(jci_ham *h, int W)
         a.tile_off = h->d_tile_off[wi];
         a.m1 = mp;
         a.W = Wd;
+        a.A = 0;
         a.ntiles = toff[D.nblk];
         if (Wd == W) h->fast_w = wi;
     }
@@ -587,6 +588,42 @@ JCI_API void jci_ham_destroy(jci_ham *h)
 }
 
 JCI_API long long jci_ham_npairs(const jci_ham *h) { return h->nnz; }
+
+JCI_API long long jci_ham_string_excitations(jci_ham *h, int spin, long long string, int kind,
+                                             long long cap, long long *targets, int *info,
+                                             double *vals)
+{
+    if (!h || !h->have_fast) { jci_set_error("string_excitations: no excitation tables"); return -1; }
+    if ((spin != 0 && spin != 1) || (kind != 1 && kind != 2)) {
+        jci_set_error("string_excitations: spin 0/1 and kind 1/2");
+        return -1;
+    }
+    const SpaceDesc &D = h->sp->h;
+    const long long n = D.s[spin].gstart[D.s[spin].ngrp];
+    if (string < 0 || string >= n) { jci_set_error("string_excitations: string out of range"); return -1; }
+    jci_ctx *c = h->sp->ctx;
+    JCI_CHECK(hipSetDevice(c->device));
+    const long long *off = kind == 1 ? h->d_off1[spin] : h->d_off2[spin];
+    long long b[2];
+    JCI_CHECK(hipMemcpyAsync(b, off + string, sizeof(b), hipMemcpyDeviceToHost, c->stream));
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    const long long cnt = b[1] - b[0], m = std::min(cnt, cap);
+    if (m > 0 && (targets || info || vals)) {
+        std::vector<int2> e(m);
+        std::vector<double> v(m);
+        JCI_CHECK(hipMemcpyAsync(e.data(), (kind == 1 ? h->d_e1[spin] : h->d_e2[spin]) + b[0],
+                                 sizeof(int2) * m, hipMemcpyDeviceToHost, c->stream));
+        JCI_CHECK(hipMemcpyAsync(v.data(), (kind == 1 ? h->d_v1[spin] : h->d_v2[spin]) + b[0],
+                                 sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+        JCI_CHECK(hipStreamSynchronize(c->stream));
+        for (long long t = 0; t < m; t++) {
+            if (targets) targets[t] = e[t].x;
+            if (info) info[t] = e[t].y;
+            if (vals) vals[t] = v[t];
+        }
+    }
+    return cnt;
+}
 
 JCI_API int jci_ham_hdiag(jci_ham *h, double *out)
 {
@@ -989,13 +1026,19 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
     double best_w_ms = 1e300;
     int best_w = h->fast_w;
     if (h->have_fast) {
-        for (int wi = 0; wi < 3; wi++) {
-            h->fast_w = wi;
-            double ms = 0;
-            RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST64, nvec, reps, &ms));
-            if (ms < best_w_ms) { best_w_ms = ms; best_w = wi; }
-        }
+        // beta tile width x alpha chunk of the table kernel
+        int best_a = 0;
+        const int chunks[3] = {1, 2, 4};
+        for (int wi = 0; wi < 3; wi++)
+            for (int ai = 0; ai < 3; ai++) {
+                h->fast_w = wi;
+                for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = chunks[ai];
+                double ms = 0;
+                RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST64, nvec, reps, &ms));
+                if (ms < best_w_ms) { best_w_ms = ms; best_w = wi; best_a = chunks[ai]; }
+            }
         h->fast_w = best_w;
+        for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = best_a;
         t[JCI_SIGMA_FAST64] = best_w_ms;
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST32, nvec, reps, &t[JCI_SIGMA_FAST32]));
     }
@@ -1040,6 +1083,26 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
     if (best64) *best64 = b64;
     if (best32) *best32 = b32;
     if (times) for (int m = 0; m <= JCI_SIGMA_GEMM32; m++) times[m] = t[m];
+    return 0;
+}
+
+JCI_API int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk)
+{
+    if (!h->have_fast) { jci_set_error("ham: no excitation tables"); return -1; }
+    if (width && *width > 0) {
+        int wi = *width == 64 ? 0 : *width == 128 ? 1 : *width == 256 ? 2 : -1;
+        if (wi < 0) { jci_set_error("ham: tile width %d not available (64, 128, 256)", *width); return -1; }
+        h->fast_w = wi;
+    }
+    if (chunk && *chunk >= 0) {
+        if (*chunk != 0 && *chunk != 1 && *chunk != 2 && *chunk != 4) {
+            jci_set_error("ham: alpha chunk %d not available (0, 1, 2, 4)", *chunk);
+            return -1;
+        }
+        for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = *chunk;
+    }
+    if (width) *width = h->fast[h->fast_w].W;
+    if (chunk) *chunk = h->fast[h->fast_w].A;
     return 0;
 }
 
@@ -1170,17 +1233,18 @@ struct Work {
     SolveOp *op;
     jci_ctx *c;
     long long N, ld;
-    double *V, *S, *T, *X, *Y;   // V,S: krymax vectors; T: JCI_MAXK vectors; X,Y: [N][JCI_MAXK]
+    int kb;                      // vectors per block: new directions per iteration, sigma chunk
+    double *V, *S, *T, *X, *Y;   // V,S: krymax vectors; T: kb vectors; X,Y: [N][kb]
     double sigma_time;
     long long ns64, ns32;
 };
 
-// S[first .. first+cnt) = H V[first .. first+cnt), in chunks of JCI_MAXK
+// S[first .. first+cnt) = H V[first .. first+cnt), in chunks of w.kb
 static int sigma_block(Work &w, int first, int cnt, int method)
 {
     double t0 = now_s();
-    for (int b = first; b < first + cnt; b += JCI_MAXK) {
-        int k = std::min(JCI_MAXK, first + cnt - b);
+    for (int b = first; b < first + cnt; b += w.kb) {
+        int k = std::min(w.kb, first + cnt - b);
         if (w.op->sigma_cols) {
             int rc = w.op->sigma_cols(w.op->self, method, k, w.V + (size_t)b * w.ld, w.ld,
                                       w.S + (size_t)b * w.ld, w.ld);
@@ -1407,10 +1471,13 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     Work w{};
     w.op = op; w.c = c; w.N = N;
     w.ld = (N + 63) / 64 * 64;
+    // block of the new directions of one iteration (ntrack at most) and of the sigma chunks:
+    // the tracked pairs, at least four (JCI_MAXK at most)
+    w.kb = std::min(JCI_MAXK, std::max(ntrack, 4));
     size_t vb = sizeof(double) * (size_t)w.ld;
     if (hipMalloc(&w.V, vb * krymax) != hipSuccess || hipMalloc(&w.S, vb * krymax) != hipSuccess ||
-        hipMalloc(&w.T, vb * JCI_MAXK) != hipSuccess || hipMalloc(&w.X, vb * JCI_MAXK) != hipSuccess ||
-        hipMalloc(&w.Y, vb * JCI_MAXK) != hipSuccess) {
+        hipMalloc(&w.T, vb * w.kb) != hipSuccess || hipMalloc(&w.X, vb * w.kb) != hipSuccess ||
+        hipMalloc(&w.Y, vb * w.kb) != hipSuccess) {
         hipFree(w.V); hipFree(w.S); hipFree(w.T); hipFree(w.X); hipFree(w.Y);
         jci_set_error("solve: out of device memory for %d basis vectors of %lld", krymax, N);
         return -3;
@@ -1436,12 +1503,12 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
         // buffer pairs), and the vectors beyond the caller's come from the block guess
         const int nsup = std::max(0, std::min(p->krymin, krymin));
         for (int i = 0; i < nsup; i++) {
-            if (hipMemcpyAsync(w.T + (size_t)(i % JCI_MAXK) * w.ld, guess + (size_t)i * N,
+            if (hipMemcpyAsync(w.T + (size_t)(i % w.kb) * w.ld, guess + (size_t)i * N,
                                sizeof(double) * N, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
                 rc = -2; goto done;
             }
-            if (i % JCI_MAXK == JCI_MAXK - 1 || i == nsup - 1) {
-                int k = i % JCI_MAXK + 1, kept;
+            if (i % w.kb == w.kb - 1 || i == nsup - 1) {
+                int k = i % w.kb + 1, kept;
                 if ((rc = orthonormalize_new(w, m, k, &kept))) goto done;
                 if (hipMemcpyAsync(w.V + (size_t)m * w.ld, w.T, vb * kept, hipMemcpyDeviceToDevice,
                                    c->stream) != hipSuccess) { rc = -2; goto done; }
@@ -1464,7 +1531,7 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
             hipFree(scr);
             if (rc) goto done;
             for (int i = 0; i < krymin && m < krymin;) {
-                const int k = std::min(JCI_MAXK, krymin - i);
+                const int k = std::min(w.kb, krymin - i);
                 int kept;
                 if (hipMemcpyAsync(w.T, w.S + (size_t)i * w.ld, vb * k, hipMemcpyDeviceToDevice,
                                    c->stream) != hipSuccess) { rc = -2; goto done; }
@@ -1640,8 +1707,8 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
         // the lowest nv Ritz vectors, JCI_MAXK at a time through T; rows past the basis
         // size (nout > m) are zero
         const int nv = std::max(nroots, nout);
-        for (int r0 = 0; r0 < nv; r0 += JCI_MAXK) {
-            const int kb = std::min(JCI_MAXK, std::min(nv, m) - r0);
+        for (int r0 = 0; r0 < nv; r0 += w.kb) {
+            const int kb = std::min(w.kb, std::min(nv, m) - r0);
             if (kb <= 0) break;
             std::vector<double> Yc((size_t)m * kb);
             for (int i = 0; i < m; i++)

@@ -23,6 +23,7 @@ struct FastArgs {
     const long long *tile_off;   // per block first tile, nblk+1 (device)
     int m1;                      // orbital pairs n(n+1)/2
     int W;                       // beta strings per tile (64, 128 or 256)
+    int A;                       // alpha singles per staged chunk (1, 2 or 4; 0 = by LDS size)
     long long ntiles;
 };
 

@@ -7,7 +7,7 @@
 //
 // Excitation tables (one per distinct string set), built once per Hamiltonian:
 //   e1[off1[I] .. off1[I+1])  single excitations of string I to strings of the space:
-//       {J, info}, info = i | f<<6 | neg<<12 | group(J)<<13
+//       {J, info}, info = i | f<<6 | neg<<12 | group(J)<<13 | tri(i,f)<<18
 //       v1 = h_if + sum_{k in I, k != i} [(kk|if) - (ki|kf)]   (one-spin part, unsigned)
 //   e2[off2[I] .. off2[I+1])  double excitations: {J, info},
 //       info = i0 | i1<<6 | f0<<12 | f1<<18 | neg<<24 | group(J)<<25
@@ -92,7 +92,8 @@ __global__ void k_exc(const SpaceDesc *__restrict__ d, int spin, long long nstr,
             if (FILL) {
                 long long p1 = b1 + c1[g];
                 int neg = jci_sign(m, i, f) < 0;
-                e1[p1] = make_int2((int)J, i | (f << 6) | (neg << 12) | (g << 13));
+                e1[p1] = make_int2((int)J, i | (f << 6) | (neg << 12) | (g << 13) |
+                                               ((int)jci_tri(i, f) << 18));
                 double v = jci_h1(m1, i, f);
                 for (uint64_t z = m & ~(1ULL << i); z; z &= z - 1) {
                     int k = jci_ctz(z);
@@ -217,16 +218,26 @@ __device__ __forceinline__ void load_space_lds(const SpaceDesc *__restrict__ d, 
     if (tid == 0) { L->GA = d->s[0].ngrp; L->GB = d->s[1].ngrp; }
 }
 
-template <typename AT, typename XT, int K, int W>
+// Alpha singles are taken A at a time: the A integral rows (P_a | .) are staged in LDS
+// together, and each beta-list entry a lane loads serves all A of them (A independent
+// gathers of x per entry instead of one entry load per gather).  Singles whose target
+// group pairs with no beta group reachable from q are skipped without staging.
+__device__ __forceinline__ int tri32(int a, int b)
+{
+    return a >= b ? ((a * (a + 1)) >> 1) + b : ((b * (b + 1)) >> 1) + a;
+}
+
+template <typename AT, typename XT, int K, int W, int A>
 __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restrict__ x,
                                                   double *__restrict__ y)
 {
     extern __shared__ unsigned char smem_raw[];
     __shared__ SpaceLds L;
-    AT *g = (AT *)smem_raw;              // staged integral row (i f | .), m1 entries
-    AT *Fa = g + a.m1;                   // sum_{k in Ia} (kk | .), m1 entries
+    AT *g = (AT *)smem_raw;              // A staged integral rows (P_a | .), m1 entries each;
+                                         // before the alpha singles: Fa
     const SpaceDesc *__restrict__ d = a.d;
     const int tid = threadIdx.x;
+    const int m1 = a.m1;
     const long long tile = (long long)blockIdx.x;
     const int b = tile_block(a.tile_off, d->nblk, tile);
     const int p = d->bp[b], q = d->bq[b];
@@ -244,13 +255,13 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     load_space_lds(d, &L, tid, W);
 
     // Fa[P] = sum over alpha electrons k of (kk|P)
-    for (int P = tid; P < a.m1; P += W) {
+    for (int P = tid; P < m1; P += W) {
         double s = 0.0;
         for (uint64_t z = ma; z; z &= z - 1) {
             int k = jci_ctz(z);
-            s += a.m2sq[(long long)jci_tri(k, k) * a.m1 + P];
+            s += a.m2sq[(long long)tri32(k, k) * m1 + P];
         }
-        Fa[P] = (AT)s;
+        g[P] = (AT)s;
     }
     __syncthreads();
 
@@ -276,8 +287,7 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
             const long long base = L.boff[bk] + ia_loc * L.gnB[gq] - L.gsB[gq];
             for (long long e = o1 + go1[gq]; e < o1 + go1[gq + 1]; e++) {
                 int2 en = TB.e1[e];
-                int i = en.y & 63, f = (en.y >> 6) & 63;
-                AT v = (AT)TB.v1[e] + Fa[jci_tri(i, f)];
+                AT v = (AT)TB.v1[e] + g[(en.y >> 18) & 4095];
                 if (en.y & (1 << 12)) v = -v;
                 long long J = base + en.x;
 #pragma unroll
@@ -309,44 +319,82 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
         }
     }
 
-    // alpha singles (uniform): own element and products with beta singles
+    // alpha singles, A at a time within each target group: own element (with the
+    // cross-spin term over the beta electrons) and products with the beta singles
+    const long long oa1 = TA.off1[Ia];
+    const int *goa1 = TA.goff1 + Ia * (GA + 1);
     const long long ob1 = TB.off1[Ib];
     const int *gob1 = TB.goff1 + Ib * (GB + 1);
-    for (long long e = TA.off1[Ia]; e < TA.off1[Ia + 1]; e++) {
-        int2 en = TA.e1[e];
-        int i = en.y & 63, f = (en.y >> 6) & 63, gp = (en.y >> 13) & 31;
-        bool neg = en.y & (1 << 12);
-        long long P = jci_tri(i, f);
-        __syncthreads();
-        for (int Q = tid; Q < a.m1; Q += W) g[Q] = (AT)a.m2sq[P * a.m1 + Q];
-        __syncthreads();
-        if (!active) continue;
-        long long ja_loc = en.x - L.gsA[gp];
-        int bk = L.blkid[gp][q];
-        if (bk >= 0) {
-            AT cross = (AT)0;
-            for (uint64_t z = mb; z; z &= z - 1) {
-                int k = jci_ctz(z);
-                cross += g[jci_tri(k, k)];
+    for (int gp = 0; gp < GA; gp++) {
+        const int bown = L.blkid[gp][q];
+        bool any = bown >= 0;
+        for (int gq = 0; gq < GB; gq++) any = any || L.blkid[gp][gq] >= 0;
+        if (!any) continue;
+        const long long s1 = oa1 + goa1[gp + 1];
+        for (long long c0 = oa1 + goa1[gp]; c0 < s1; c0 += A) {
+            const int na = (int)(s1 - c0 < A ? s1 - c0 : A);
+            long long jl[A];
+            int ng[A];
+            AT v1[A];
+#pragma unroll
+            for (int t = 0; t < A; t++) {
+                const int2 en = TA.e1[c0 + (t < na ? t : 0)];
+                jl[t] = en.x - L.gsA[gp];
+                ng[t] = (en.y >> 12) & 1;
+                v1[t] = (AT)TA.v1[c0 + (t < na ? t : 0)];
             }
-            AT v = (AT)TA.v1[e] + cross;
-            if (neg) v = -v;
-            long long J = L.boff[bk] + ja_loc * nq + ib_loc;
+            __syncthreads();
+            for (int t = 0; t < na; t++) {
+                const long long P = (TA.e1[c0 + t].y >> 18) & 4095;
+                for (int Q = tid; Q < m1; Q += W) g[t * m1 + Q] = (AT)a.m2sq[P * m1 + Q];
+            }
+            __syncthreads();
+            if (!active) continue;
+            if (bown >= 0) {
 #pragma unroll
-            for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
-        }
-        for (int gq = 0; gq < GB; gq++) {
-            int bk2 = L.blkid[gp][gq];
-            if (bk2 < 0) continue;
-            const long long base = L.boff[bk2] + ja_loc * L.gnB[gq] - L.gsB[gq];
-            for (long long e2 = ob1 + gob1[gq]; e2 < ob1 + gob1[gq + 1]; e2++) {
-                int2 eb = TB.e1[e2];
-                int kk = eb.y & 63, ll = (eb.y >> 6) & 63;
-                AT v = g[jci_tri(kk, ll)];
-                if (neg != (bool)(eb.y & (1 << 12))) v = -v;
-                long long J = base + eb.x;
+                for (int t = 0; t < A; t++) {
+                    if (t >= na) break;
+                    AT cross = (AT)0;
+                    for (uint64_t z = mb; z; z &= z - 1) {
+                        int k = jci_ctz(z);
+                        cross += g[t * m1 + tri32(k, k)];
+                    }
+                    AT v = v1[t] + cross;
+                    if (ng[t]) v = -v;
+                    const long long J = L.boff[bown] + jl[t] * nq + ib_loc;
 #pragma unroll
-                for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+                    for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
+                }
+            }
+            for (int gq = 0; gq < GB; gq++) {
+                const int bk2 = L.blkid[gp][gq];
+                if (bk2 < 0) continue;
+                long long base[A];
+#pragma unroll
+                for (int t = 0; t < A; t++) base[t] = L.boff[bk2] + jl[t] * L.gnB[gq] - L.gsB[gq];
+                for (long long e2 = ob1 + gob1[gq]; e2 < ob1 + gob1[gq + 1]; e2++) {
+                    const int2 eb = TB.e1[e2];
+                    const int Q = (eb.y >> 18) & 4095;
+                    const int nb = (eb.y >> 12) & 1;
+                    XT xv[A][K];
+#pragma unroll
+                    for (int t = 0; t < A; t++) {
+                        if (t < na) {
+                            const XT *xp = x + (base[t] + eb.x) * K;
+#pragma unroll
+                            for (int k = 0; k < K; k++) xv[t][k] = xp[k];
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < A; t++) {
+                        if (t < na) {
+                            AT v = g[t * m1 + Q];
+                            if (ng[t] != nb) v = -v;
+#pragma unroll
+                            for (int k = 0; k < K; k++) acc[k] += v * (AT)xv[t][k];
+                        }
+                    }
+                }
             }
         }
     }
@@ -356,24 +404,38 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     }
 }
 
-template <typename AT, typename XT, int W>
-static int launch_fast(const FastArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
+// alpha singles per staged chunk: 4 while four integral rows fit 64 KB of LDS, else 2
+static inline int fast_chunk(int m1, size_t esz) { return 4 * (size_t)m1 * esz <= 65536 ? 4 : 2; }
+
+template <typename AT, typename XT, int W, int A>
+static int launch_fast_a(const FastArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
 {
-    size_t shm = sizeof(AT) * 2 * (size_t)a.m1;
+    size_t shm = sizeof(AT) * (size_t)A * (size_t)a.m1;
     dim3 g((unsigned)a.ntiles), b(W);
     switch (nvec) {
-    case 1: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 1, W>), g, b, shm, st, a, x, y); break;
-    case 2: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 2, W>), g, b, shm, st, a, x, y); break;
-    case 3: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 3, W>), g, b, shm, st, a, x, y); break;
-    case 4: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 4, W>), g, b, shm, st, a, x, y); break;
-    case 5: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 5, W>), g, b, shm, st, a, x, y); break;
-    case 6: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 6, W>), g, b, shm, st, a, x, y); break;
-    case 7: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 7, W>), g, b, shm, st, a, x, y); break;
-    case 8: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 8, W>), g, b, shm, st, a, x, y); break;
+    case 1: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 1, W, A>), g, b, shm, st, a, x, y); break;
+    case 2: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 2, W, A>), g, b, shm, st, a, x, y); break;
+    case 3: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 3, W, A>), g, b, shm, st, a, x, y); break;
+    case 4: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 4, W, A>), g, b, shm, st, a, x, y); break;
+    case 5: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 5, W, A>), g, b, shm, st, a, x, y); break;
+    case 6: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 6, W, A>), g, b, shm, st, a, x, y); break;
+    case 7: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 7, W, A>), g, b, shm, st, a, x, y); break;
+    case 8: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 8, W, A>), g, b, shm, st, a, x, y); break;
     default: jci_set_error("sigma_fast: nvec must be 1..8"); return -1;
     }
     LAUNCH_CHECK2();
     return 0;
+}
+
+template <typename AT, typename XT, int W>
+static int launch_fast(const FastArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
+{
+    const int A = a.A > 0 ? a.A : fast_chunk(a.m1, sizeof(AT));
+    if (A == 4) return launch_fast_a<AT, XT, W, 4>(a, nvec, x, y, st);
+    if (A == 2) return launch_fast_a<AT, XT, W, 2>(a, nvec, x, y, st);
+    if (A == 1) return launch_fast_a<AT, XT, W, 1>(a, nvec, x, y, st);
+    jci_set_error("sigma_fast: alpha chunk %d not available (1, 2 or 4)", A);
+    return -1;
 }
 
 int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *y, hipStream_t st)
