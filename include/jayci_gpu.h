@@ -334,6 +334,11 @@ int jci_ham_fast_width(const jci_ham *h);
  * most that fit 64 KB of LDS, at most 4); positive values set, -1 / 0 keep (width) or
  * choose automatically (chunk 0); both are returned */
 int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk);
+/* *split 1: the table sigma runs as two passes, the alpha side and the alpha-beta products,
+ * then the beta singles and doubles over a block-transposed copy of the input (N * nvec
+ * more device elements); 0: one fused kernel.  -1 keeps; the value in use is returned
+ * (default 1 for spaces of more than one block; the autotuner sets it) */
+int jci_ham_fast_split(jci_ham *h, int *split);
 /* column-tile widths of the dense-string sigma's opposite-spin product (FP64: 16 or 32,
  * FP32: 32 or 64; the autotuner sets them): a positive *bn sets, then both are returned */
 int jci_ham_gemm_tiles(jci_ham *h, int *bn64, int *bn32);

@@ -92,6 +92,7 @@ _sig("jci_bench_copy", _i, [_P, _ll, _i, ctypes.POINTER(_d)])
 _sig("jci_ham_autotune", _i, [_P, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _P])
 _sig("jci_ham_fast_width", _i, [_P])
 _sig("jci_ham_fast_tiles", _i, [_P, ctypes.POINTER(_i), ctypes.POINTER(_i)])
+_sig("jci_ham_fast_split", _i, [_P, ctypes.POINTER(_i)])
 _sig("jci_batch_create", _i, [_P, _i, _i, ctypes.POINTER(_P)])
 _sig("jci_batch_destroy", None, [_P])
 _sig("jci_batch_set_graphs", _i, [_P, _i])
@@ -379,6 +380,12 @@ class Hamiltonian:
         w, c = _i(width), _i(chunk)
         _check(_L.jci_ham_fast_tiles(self._h, ctypes.byref(w), ctypes.byref(c)), "jci_ham_fast_tiles")
         return w.value, c.value
+
+    def fast_split(self, split=-1):
+        """Two-pass table sigma on (1) or off (0); -1 queries (jci_ham_fast_split)."""
+        v = _i(split)
+        _check(_L.jci_ham_fast_split(self._h, ctypes.byref(v)), "jci_ham_fast_split")
+        return v.value
 
     def gemm_tiles(self, bn64=0, bn32=0):
         """E tile widths of the dense-string sigma; positive arguments set them."""

@@ -264,6 +264,14 @@ struct jci_ham {
     long long *d_tile_off[3];   // tile maps for W = 64, 128, 256
     FastArgs fast[3];
     int fast_w;                 // index into fast[] used by the FAST methods
+    // the beta side as a second pass over the block-transposed vector (jk_sigma_fast_split)
+    int fast_split;             // 1: two passes, 0: one fused kernel
+    SpaceDesc *d_descT;         // transposed descriptor (spins swapped)
+    long long *d_tileT_off[3];
+    FastArgs fastT[3];
+    long long *d_tt_off, ntt;   // 16 x 16 transpose tiles per block
+    void *d_xT;                 // block-transposed input
+    size_t xT_bytes;
     int have_fci;               // single-block space with n <= 16: FCI kernel usable
     FciArgs fci;
     int tuned64, tuned32;       // autotuned default methods (0 = not tuned)
@@ -506,6 +514,45 @@ static int build_fast(jci_ham *h, int W)
         a.ntiles = toff[D.nblk];
         if (Wd == W) h->fast_w = wi;
     }
+    // transposed space for the beta-side pass
+    {
+        SpaceDesc T;
+        jci_transpose_desc(D, &T);
+        JCI_CHECK(hipMalloc(&h->d_descT, sizeof(SpaceDesc)));
+        JCI_CHECK(hipMemcpyAsync(h->d_descT, &T, sizeof(SpaceDesc), hipMemcpyHostToDevice, c->stream));
+        std::vector<long long> tt(D.nblk + 1, 0);
+        for (int b = 0; b < D.nblk; b++) {
+            long long na = D.s[0].gn[D.bp[b]], nq = D.s[1].gn[D.bq[b]];
+            tt[b + 1] = tt[b] + ((na + 15) / 16) * ((nq + 15) / 16);
+        }
+        h->ntt = tt[D.nblk];
+        JCI_CHECK(hipMalloc(&h->d_tt_off, sizeof(long long) * (D.nblk + 1)));
+        JCI_CHECK(hipMemcpyAsync(h->d_tt_off, tt.data(), sizeof(long long) * (D.nblk + 1),
+                                 hipMemcpyHostToDevice, c->stream));
+        for (int wi = 0; wi < 3; wi++) {
+            const int Wd = widths[wi];
+            std::vector<long long> toff(D.nblk + 1, 0);
+            for (int b = 0; b < D.nblk; b++) {
+                long long nbq = D.s[1].gn[D.bq[b]], na = D.s[0].gn[D.bp[b]];
+                toff[b + 1] = toff[b] + nbq * ((na + Wd - 1) / Wd);
+            }
+            JCI_CHECK(hipMalloc(&h->d_tileT_off[wi], sizeof(long long) * (D.nblk + 1)));
+            JCI_CHECK(hipMemcpyAsync(h->d_tileT_off[wi], toff.data(), sizeof(long long) * (D.nblk + 1),
+                                     hipMemcpyHostToDevice, c->stream));
+            JCI_CHECK(hipStreamSynchronize(c->stream));
+            FastArgs &t = h->fastT[wi];
+            t = h->fast[wi];
+            t.d = h->d_descT;
+            t.ma = s->d_mask[1];
+            t.mb = s->d_mask[0];
+            t.ta = h->fast[wi].tb;
+            t.tb = h->fast[wi].ta;
+            t.hd = nullptr;
+            t.tile_off = h->d_tileT_off[wi];
+            t.ntiles = toff[D.nblk];
+        }
+        h->fast_split = D.nblk > 1;
+    }
     h->have_fast = 1;
     if (D.nblk == 1 && D.norbs <= FCI_MAXORB) {
         FciArgs &f = h->fci;
@@ -572,6 +619,10 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     }
     hipFree(h->d_m2sq);
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tile_off[wi]);
+    for (int wi = 0; wi < 3; wi++) hipFree(h->d_tileT_off[wi]);
+    hipFree(h->d_descT);
+    hipFree(h->d_tt_off);
+    hipFree(h->d_xT);
     if (h->d_gfH64[1] != h->d_gfH64[0]) { hipFree(h->d_gfH64[1]); hipFree(h->d_gfH32[1]); }
     hipFree(h->d_gfH64[0]); hipFree(h->d_gfH32[0]);
     hipFree(h->d_gfM64); hipFree(h->d_gfM32); hipFree(h->d_gfla); hipFree(h->d_gflb);
@@ -878,13 +929,27 @@ static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *
         return jk_sigma_generic(32, nvec, sp->d_desc, N, sp->d_mask[0], sp->d_mask[1], c->d_binom,
                                 h->d_m1, h->d_m2, h->d_hd, xf, y, c->stream);
     case JCI_SIGMA_FAST64:
+    case JCI_SIGMA_FAST32: {
         if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
-        return jk_sigma_fast(64, nvec, h->fast[h->fast_w], x, y, c->stream);
-    case JCI_SIGMA_FAST32:
-        if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
-        RCHECK(ensure_xf(h, N * nvec, s, &xf));
-        RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
-        return jk_sigma_fast(32, nvec, h->fast[h->fast_w], xf, y, c->stream);
+        const int prec = method == JCI_SIGMA_FAST64 ? 64 : 32;
+        const void *xin = x;
+        if (prec == 32) {
+            RCHECK(ensure_xf(h, N * nvec, s, &xf));
+            RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
+            xin = xf;
+        }
+        if (!h->fast_split) return jk_sigma_fast(prec, nvec, h->fast[h->fast_w], xin, y, c->stream);
+        const size_t need = (size_t)N * nvec * (prec == 64 ? sizeof(double) : sizeof(float));
+        void *xT;
+        if (s) {
+            RCHECK(jci_scratch_carve(s, need, &xT, "the transposed input"));
+        } else {
+            RCHECK(grow(&h->d_xT, &h->xT_bytes, need, "the transposed input"));
+            xT = h->d_xT;
+        }
+        return jk_sigma_fast_split(prec, nvec, h->fast[h->fast_w], h->fastT[h->fast_w],
+                                   h->d_tt_off, h->ntt, xin, xT, y, c->stream);
+    }
     case JCI_SIGMA_FCI64:
         if (!h->have_fci) { jci_set_error("sigma: single-block kernel not applicable"); return -1; }
         return jk_sigma_fci(64, nvec, h->fci, x, y, c->stream);
@@ -1027,17 +1092,22 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
     int best_w = h->fast_w;
     if (h->have_fast) {
         // beta tile width x alpha chunk of the table kernel
-        int best_a = 0;
+        int best_a = 0, best_split = h->fast_split;
         const int chunks[3] = {1, 2, 4};
-        for (int wi = 0; wi < 3; wi++)
-            for (int ai = 0; ai < 3; ai++) {
-                h->fast_w = wi;
-                for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = chunks[ai];
-                double ms = 0;
-                RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST64, nvec, reps, &ms));
-                if (ms < best_w_ms) { best_w_ms = ms; best_w = wi; best_a = chunks[ai]; }
-            }
+        for (int sp = 0; sp < 2; sp++)
+            for (int wi = 0; wi < 3; wi++)
+                for (int ai = 0; ai < 3; ai++) {
+                    h->fast_split = sp;
+                    h->fast_w = wi;
+                    for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = chunks[ai];
+                    double ms = 0;
+                    RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST64, nvec, reps, &ms));
+                    if (ms < best_w_ms) {
+                        best_w_ms = ms; best_w = wi; best_a = chunks[ai]; best_split = sp;
+                    }
+                }
         h->fast_w = best_w;
+        h->fast_split = best_split;
         for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = best_a;
         t[JCI_SIGMA_FAST64] = best_w_ms;
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST32, nvec, reps, &t[JCI_SIGMA_FAST32]));
@@ -1103,6 +1173,14 @@ JCI_API int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk)
     }
     if (width) *width = h->fast[h->fast_w].W;
     if (chunk) *chunk = h->fast[h->fast_w].A;
+    return 0;
+}
+
+JCI_API int jci_ham_fast_split(jci_ham *h, int *split)
+{
+    if (!h->have_fast) { jci_set_error("ham: no excitation tables"); return -1; }
+    if (split && *split >= 0) h->fast_split = *split ? 1 : 0;
+    if (split) *split = h->fast_split;
     return 0;
 }
 

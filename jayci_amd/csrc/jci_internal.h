@@ -45,6 +45,9 @@ struct SpaceDesc {
 int jci_build_desc(int na, int nb, int norbs, int ndocc, int nactv, int xlvl, int rule,
                    SpaceDesc *d);
 void jci_binom_table(unsigned long long *tab /* 65*65 */);
+// the space with alpha and beta swapped (blocks in the same order and at the same offsets,
+// each stored beta-major)
+void jci_transpose_desc(const SpaceDesc &D, SpaceDesc *T);
 
 #define JCI_API extern "C" __attribute__((visibility("default")))
 

@@ -50,3 +50,10 @@ int jk_exc_fill(const SpaceDesc *d, int spin, long long nstr, const uint64_t *M,
 int jk_m2_square(int m1, const double *m2, double *sq, hipStream_t st);
 int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *y,
                   hipStream_t st);
+// the same sigma in two passes: the alpha side, diagonal and alpha-beta products on the
+// space (a), then the beta singles and doubles on the transposed space (t: descriptor from
+// jci_transpose_desc, spins and tables swapped) over xT, the block-transposed copy of x
+// (nvec * N elements of the input's type; tt_off / ntt: 16 x 16 transpose tiles per block)
+int jk_sigma_fast_split(int prec, int nvec, const FastArgs &a, const FastArgs &t,
+                        const long long *tt_off, long long ntt, const void *x, void *xT,
+                        double *y, hipStream_t st);

@@ -227,7 +227,7 @@ __device__ __forceinline__ int tri32(int a, int b)
     return a >= b ? ((a * (a + 1)) >> 1) + b : ((b * (b + 1)) >> 1) + a;
 }
 
-template <typename AT, typename XT, int K, int W, int A>
+template <typename AT, typename XT, int K, int W, int A, bool BSIDE>
 __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restrict__ x,
                                                   double *__restrict__ y)
 {
@@ -255,13 +255,15 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     load_space_lds(d, &L, tid, W);
 
     // Fa[P] = sum over alpha electrons k of (kk|P)
-    for (int P = tid; P < m1; P += W) {
-        double s = 0.0;
-        for (uint64_t z = ma; z; z &= z - 1) {
-            int k = jci_ctz(z);
-            s += a.m2sq[(long long)tri32(k, k) * m1 + P];
+    if (BSIDE) {
+        for (int P = tid; P < m1; P += W) {
+            double s = 0.0;
+            for (uint64_t z = ma; z; z &= z - 1) {
+                int k = jci_ctz(z);
+                s += a.m2sq[(long long)tri32(k, k) * m1 + P];
+            }
+            g[P] = (AT)s;
         }
-        g[P] = (AT)s;
     }
     __syncthreads();
 
@@ -277,8 +279,9 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     const ExcView &TA = a.ta, &TB = a.tb;
     const int GA = L.GA, GB = L.GB;
 
-    // beta excitations of Ib (per thread), only towards groups paired with p
-    if (active) {
+    // beta excitations of Ib (per thread), only towards groups paired with p (without
+    // BSIDE the transposed pass k_sigma_fast_beta adds them)
+    if (BSIDE && active) {
         const long long o1 = TB.off1[Ib], o2 = TB.off2[Ib];
         const int *go1 = TB.goff1 + Ib * (GB + 1), *go2 = TB.goff2 + Ib * (GB + 1);
         for (int gq = 0; gq < GB; gq++) {
@@ -407,50 +410,239 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
 // alpha singles per staged chunk: 4 while four integral rows fit 64 KB of LDS, else 2
 static inline int fast_chunk(int m1, size_t esz) { return 4 * (size_t)m1 * esz <= 65536 ? 4 : 2; }
 
-template <typename AT, typename XT, int W, int A>
+// ---- the beta side as a pass over the transposed vector ------------------------------------
+// With the space transposed (spins swapped: blocks stored beta-major, see
+// jci_transpose_desc), the beta singles' own elements and the beta doubles of the original
+// space are the alpha-side work of the transposed one: uniform over a work-group (its
+// beta string), with coalesced reads of the transposed vector.  The result is added to the
+// original-layout sigma.  Cross-spin term of a single i -> f: sum over the other spin's
+// electrons k of (if|kk), read from row tri(i,f) of the square integrals.
+template <typename AT, typename XT, int K, int W>
+__global__ __launch_bounds__(W) void k_sigma_fast_beta(FastArgs a, const XT *__restrict__ xT,
+                                                       double *__restrict__ y)
+{
+    __shared__ SpaceLds L;
+    const SpaceDesc *__restrict__ d = a.d;          // transposed descriptor
+    const int tid = threadIdx.x;
+    const int m1 = a.m1;
+    const long long tile = (long long)blockIdx.x;
+    const int b = tile_block(a.tile_off, d->nblk, tile);
+    const int p = d->bp[b], q = d->bq[b];           // beta group, alpha group
+    const long long nq = d->s[1].gn[q];             // alpha strings of the block
+    const long long nch = (nq + W - 1) / W;
+    const long long rel = tile - a.tile_off[b];
+    const long long ia_loc = rel / nch, chunk = rel - ia_loc * nch;   // beta string
+    const long long ib_loc = chunk * W + tid;                         // alpha string
+    const bool active = ib_loc < nq;
+    const long long Ia = d->s[0].gstart[p] + ia_loc;
+    const long long Ib = d->s[1].gstart[q] + (active ? ib_loc : 0);
+    const uint64_t mb = a.mb[Ib];
+    load_space_lds(d, &L, tid, W);
+    __syncthreads();
+    if (!active) return;
+    AT acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = (AT)0;
+    const ExcView &TA = a.ta;
+    const int GA = L.GA;
+    // doubles of the (beta) string, uniform over the work-group
+    {
+        const long long o2 = TA.off2[Ia];
+        const int *go2 = TA.goff2 + Ia * (GA + 1);
+        for (int gp = 0; gp < GA; gp++) {
+            int bk = L.blkid[gp][q];
+            if (bk < 0) continue;
+            const long long base = L.boff[bk] - L.gsA[gp] * nq + ib_loc;
+            for (long long e = o2 + go2[gp]; e < o2 + go2[gp + 1]; e++) {
+                AT v = (AT)TA.v2[e];
+                long long J = base + (long long)TA.e2[e].x * nq;
+#pragma unroll
+                for (int k = 0; k < K; k++) acc[k] += v * (AT)xT[J * K + k];
+            }
+        }
+    }
+    // singles of the (beta) string: own element with the cross-spin term
+    {
+        const long long o1 = TA.off1[Ia];
+        const int *go1 = TA.goff1 + Ia * (GA + 1);
+        for (int gp = 0; gp < GA; gp++) {
+            int bk = L.blkid[gp][q];
+            if (bk < 0) continue;
+            const long long base = L.boff[bk] - L.gsA[gp] * nq + ib_loc;
+            for (long long e = o1 + go1[gp]; e < o1 + go1[gp + 1]; e++) {
+                const int2 en = TA.e1[e];
+                const double *__restrict__ row = a.m2sq + (long long)((en.y >> 18) & 4095) * m1;
+                AT cross = (AT)0;
+                for (uint64_t z = mb; z; z &= z - 1) {
+                    int k = jci_ctz(z);
+                    cross += (AT)row[tri32(k, k)];
+                }
+                AT v = (AT)TA.v1[e] + cross;
+                if (en.y & (1 << 12)) v = -v;
+                long long J = base + (long long)en.x * nq;
+#pragma unroll
+                for (int k = 0; k < K; k++) acc[k] += v * (AT)xT[J * K + k];
+            }
+        }
+    }
+    // add into the original (alpha-major) layout: determinant (alpha ib_loc, beta ia_loc)
+    const long long O = L.boff[b] + ib_loc * L.gnA[p] + ia_loc;
+#pragma unroll
+    for (int k = 0; k < K; k++) y[O * K + k] += (double)acc[k];
+}
+
+// x[N][K] -> xT[N][K] with every block transposed (alpha-major -> beta-major), 16 x 16
+// element tiles through LDS; tt_off[b] = first tile of block b
+template <typename XT, int K>
+__global__ __launch_bounds__(256) void k_block_transpose(const SpaceDesc *__restrict__ d,
+                                                         const long long *__restrict__ tt_off,
+                                                         const XT *__restrict__ x,
+                                                         XT *__restrict__ xT)
+{
+    __shared__ XT t[16][17][K];
+    const long long tile = blockIdx.x;
+    const int b = tile_block(tt_off, d->nblk, tile);
+    const long long na = d->s[0].gn[d->bp[b]], nb = d->s[1].gn[d->bq[b]];
+    const long long ntc = (nb + 15) / 16;
+    const long long rel = tile - tt_off[b];
+    const long long r0 = rel / ntc * 16, c0 = rel % ntc * 16;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const long long off = d->boff[b];
+    if (r0 + ty < na && c0 + tx < nb) {
+        const XT *src = x + (off + (r0 + ty) * nb + c0 + tx) * K;
+#pragma unroll
+        for (int k = 0; k < K; k++) t[ty][tx][k] = src[k];
+    }
+    __syncthreads();
+    if (c0 + ty < nb && r0 + tx < na) {
+        XT *dst = xT + (off + (c0 + ty) * na + r0 + tx) * K;
+#pragma unroll
+        for (int k = 0; k < K; k++) dst[k] = t[tx][ty][k];
+    }
+}
+
+template <typename AT, typename XT, int W, int A, bool BSIDE>
 static int launch_fast_a(const FastArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
 {
     size_t shm = sizeof(AT) * (size_t)A * (size_t)a.m1;
     dim3 g((unsigned)a.ntiles), b(W);
     switch (nvec) {
-    case 1: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 1, W, A>), g, b, shm, st, a, x, y); break;
-    case 2: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 2, W, A>), g, b, shm, st, a, x, y); break;
-    case 3: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 3, W, A>), g, b, shm, st, a, x, y); break;
-    case 4: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 4, W, A>), g, b, shm, st, a, x, y); break;
-    case 5: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 5, W, A>), g, b, shm, st, a, x, y); break;
-    case 6: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 6, W, A>), g, b, shm, st, a, x, y); break;
-    case 7: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 7, W, A>), g, b, shm, st, a, x, y); break;
-    case 8: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 8, W, A>), g, b, shm, st, a, x, y); break;
+    case 1: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 1, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
+    case 2: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 2, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
+    case 3: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 3, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
+    case 4: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 4, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
+    case 5: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 5, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
+    case 6: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 6, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
+    case 7: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 7, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
+    case 8: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 8, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
     default: jci_set_error("sigma_fast: nvec must be 1..8"); return -1;
     }
     LAUNCH_CHECK2();
     return 0;
 }
 
-template <typename AT, typename XT, int W>
+template <typename AT, typename XT, int W, bool BSIDE>
 static int launch_fast(const FastArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
 {
     const int A = a.A > 0 ? a.A : fast_chunk(a.m1, sizeof(AT));
-    if (A == 4) return launch_fast_a<AT, XT, W, 4>(a, nvec, x, y, st);
-    if (A == 2) return launch_fast_a<AT, XT, W, 2>(a, nvec, x, y, st);
-    if (A == 1) return launch_fast_a<AT, XT, W, 1>(a, nvec, x, y, st);
+    if (A == 4) return launch_fast_a<AT, XT, W, 4, BSIDE>(a, nvec, x, y, st);
+    if (A == 2) return launch_fast_a<AT, XT, W, 2, BSIDE>(a, nvec, x, y, st);
+    if (A == 1) return launch_fast_a<AT, XT, W, 1, BSIDE>(a, nvec, x, y, st);
     jci_set_error("sigma_fast: alpha chunk %d not available (1, 2 or 4)", A);
     return -1;
 }
 
-int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *y, hipStream_t st)
+template <bool BSIDE>
+static int fast_main(int prec, int nvec, const FastArgs &a, const void *x, double *y, hipStream_t st)
 {
     if (a.ntiles <= 0) return 0;
     if (a.W == 64) {
-        if (prec == 64) return launch_fast<double, double, 64>(a, nvec, (const double *)x, y, st);
-        return launch_fast<float, float, 64>(a, nvec, (const float *)x, y, st);
+        if (prec == 64) return launch_fast<double, double, 64, BSIDE>(a, nvec, (const double *)x, y, st);
+        return launch_fast<float, float, 64, BSIDE>(a, nvec, (const float *)x, y, st);
     }
     if (a.W == 128) {
-        if (prec == 64) return launch_fast<double, double, 128>(a, nvec, (const double *)x, y, st);
-        return launch_fast<float, float, 128>(a, nvec, (const float *)x, y, st);
+        if (prec == 64) return launch_fast<double, double, 128, BSIDE>(a, nvec, (const double *)x, y, st);
+        return launch_fast<float, float, 128, BSIDE>(a, nvec, (const float *)x, y, st);
     }
-    if (prec == 64) return launch_fast<double, double, 256>(a, nvec, (const double *)x, y, st);
-    return launch_fast<float, float, 256>(a, nvec, (const float *)x, y, st);
+    if (prec == 64) return launch_fast<double, double, 256, BSIDE>(a, nvec, (const double *)x, y, st);
+    return launch_fast<float, float, 256, BSIDE>(a, nvec, (const float *)x, y, st);
+}
+
+int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *y, hipStream_t st)
+{
+    return fast_main<true>(prec, nvec, a, x, y, st);
+}
+
+template <typename XT, int K>
+static int launch_transpose(const SpaceDesc *d, const long long *tt_off, long long ntt,
+                            const XT *x, XT *xT, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_block_transpose<XT, K>), dim3((unsigned)ntt), dim3(256), 0, st, d,
+                       tt_off, x, xT);
+    LAUNCH_CHECK2();
+    return 0;
+}
+
+template <typename XT>
+static int transpose_blocks(int nvec, const SpaceDesc *d, const long long *tt_off, long long ntt,
+                            const XT *x, XT *xT, hipStream_t st)
+{
+    switch (nvec) {
+    case 1: return launch_transpose<XT, 1>(d, tt_off, ntt, x, xT, st);
+    case 2: return launch_transpose<XT, 2>(d, tt_off, ntt, x, xT, st);
+    case 3: return launch_transpose<XT, 3>(d, tt_off, ntt, x, xT, st);
+    case 4: return launch_transpose<XT, 4>(d, tt_off, ntt, x, xT, st);
+    case 5: return launch_transpose<XT, 5>(d, tt_off, ntt, x, xT, st);
+    case 6: return launch_transpose<XT, 6>(d, tt_off, ntt, x, xT, st);
+    case 7: return launch_transpose<XT, 7>(d, tt_off, ntt, x, xT, st);
+    case 8: return launch_transpose<XT, 8>(d, tt_off, ntt, x, xT, st);
+    }
+    jci_set_error("sigma_fast: nvec must be 1..8");
+    return -1;
+}
+
+template <typename AT, typename XT, int W>
+static int launch_beta(const FastArgs &t, int nvec, const XT *xT, double *y, hipStream_t st)
+{
+    dim3 g((unsigned)t.ntiles), b(W);
+    switch (nvec) {
+    case 1: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 1, W>), g, b, 0, st, t, xT, y); break;
+    case 2: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 2, W>), g, b, 0, st, t, xT, y); break;
+    case 3: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 3, W>), g, b, 0, st, t, xT, y); break;
+    case 4: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 4, W>), g, b, 0, st, t, xT, y); break;
+    case 5: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 5, W>), g, b, 0, st, t, xT, y); break;
+    case 6: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 6, W>), g, b, 0, st, t, xT, y); break;
+    case 7: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 7, W>), g, b, 0, st, t, xT, y); break;
+    case 8: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 8, W>), g, b, 0, st, t, xT, y); break;
+    default: jci_set_error("sigma_fast: nvec must be 1..8"); return -1;
+    }
+    LAUNCH_CHECK2();
+    return 0;
+}
+
+#define RCHECK_SF(x)            \
+    do {                        \
+        int r_ = (x);           \
+        if (r_) return r_;      \
+    } while (0)
+
+int jk_sigma_fast_split(int prec, int nvec, const FastArgs &a, const FastArgs &t,
+                        const long long *tt_off, long long ntt, const void *x, void *xT,
+                        double *y, hipStream_t st)
+{
+    if (a.ntiles <= 0) return 0;
+    // alpha side, diagonal and alpha-beta products write y; the beta side adds to it
+    RCHECK_SF(fast_main<false>(prec, nvec, a, x, y, st));
+    if (prec == 64) {
+        RCHECK_SF(transpose_blocks<double>(nvec, a.d, tt_off, ntt, (const double *)x, (double *)xT, st));
+        if (t.W == 64) return launch_beta<double, double, 64>(t, nvec, (const double *)xT, y, st);
+        if (t.W == 128) return launch_beta<double, double, 128>(t, nvec, (const double *)xT, y, st);
+        return launch_beta<double, double, 256>(t, nvec, (const double *)xT, y, st);
+    }
+    RCHECK_SF(transpose_blocks<float>(nvec, a.d, tt_off, ntt, (const float *)x, (float *)xT, st));
+    if (t.W == 64) return launch_beta<float, float, 64>(t, nvec, (const float *)xT, y, st);
+    if (t.W == 128) return launch_beta<float, float, 128>(t, nvec, (const float *)xT, y, st);
+    return launch_beta<float, float, 256>(t, nvec, (const float *)xT, y, st);
 }
 
 // ---- single-block spaces (FCI) ------------------------------------------------------------

@@ -111,3 +111,19 @@ int jci_build_desc(int na, int nb, int norbs, int ndocc, int nactv, int xlvl, in
     d->ndets = off;
     return 0;
 }
+
+// The space with its spins swapped: spin 0 describes the beta strings, blocks keep their
+// order and offsets (block b holds the same determinants, stored beta-major).
+void jci_transpose_desc(const SpaceDesc &D, SpaceDesc *T)
+{
+    *T = D;
+    T->s[0] = D.s[1];
+    T->s[1] = D.s[0];
+    for (int i = 0; i < JCI_MAXG; i++)
+        for (int j = 0; j < JCI_MAXG; j++) T->blkid[i][j] = -1;
+    for (int b = 0; b < D.nblk; b++) {
+        T->bp[b] = D.bq[b];
+        T->bq[b] = D.bp[b];
+        T->blkid[D.bq[b]][D.bp[b]] = b;
+    }
+}
