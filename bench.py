@@ -57,9 +57,15 @@ def percentile(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
+LANE_NORBS = 16     # requests with more orbitals run on the serving loop's lane
+
+
 def solve_kw(J, spec, N, restol):
-    from jayci_amd.serve import solve_params
-    kw = solve_params(spec, N, restol)
+    """The request's Davidson settings, the same on every path: the batched ones for the
+    full-CI requests, the lane ones for the MRCI-SD requests."""
+    from jayci_amd.serve import solve_params, lane_solve_params
+    kw = (lane_solve_params(spec, N, restol) if spec["norbs"] > LANE_NORBS
+          else solve_params(spec, N, restol))
     kw["vectors"] = False
     return kw
 
@@ -69,6 +75,24 @@ def make_instance(J, ctx, spec, seed, flags):
     sp = J.Space(ctx, spec["na"], spec["nb"], spec["norbs"], spec["ndocc"], spec["nactv"],
                  spec["xlvl"], spec["rule"])
     return sp, J.Hamiltonian(sp, m1, m2, flags)
+
+
+class c_stdout_to_stderr:
+    """Send what C code writes to file descriptor 1 to stderr while active (the solver's
+    per-iteration lines); stdout carries only the JSON line."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
 
 
 def fci_pairs(spec):
@@ -230,6 +254,9 @@ def reference_energies(J, ctx, stream):
     1e-8 Ha.  Returns (energies, largest final residual norm)."""
     ref, rmax = [], 0.0
     for arr, s, seed in stream:
+        if s["norbs"] > LANE_NORBS:
+            ref.append(None)          # MRCI-SD requests: the plain FP64 solve is the reference
+            continue
         sp, h = make_instance(J, ctx, s, seed, J.DIRECT_TABLES)
         kw = solve_kw(J, s, sp.ndets, 1e-8)
         kw.update(nbuffer=2, krymin=min(s["nroots"] + 3, sp.ndets),
@@ -245,18 +272,35 @@ def reference_energies(J, ctx, stream):
 
 def bench_stream(J, ctx, args, rate, seed):
     from jayci_amd import serve
-    stream = serve.make_stream(args.steps, rate, seed)
+    share = args.mrci_share if rate else 0.0
+    stream = serve.make_stream(args.steps, rate, seed, mrci_share=share)
     warm = serve.make_stream(args.warmup, 0.0, seed + 1) if args.warmup else []
-    ndets = [serve.ndets_of(s) for _, s, _ in stream]
-    log("stream: %d requests, %d determinants in total, largest %d"
-        % (len(stream), sum(ndets), max(ndets)))
+    ndets = []
+    for _, s, _ in stream:
+        n = serve.ndets_of(s)
+        if n is None:
+            sp = J.Space(ctx, s["na"], s["nb"], s["norbs"], s["ndocc"], s["nactv"], s["xlvl"],
+                         s["rule"])
+            n = sp.ndets
+            sp.close()
+        ndets.append(n)
+    nmrci = sum(1 for _, s, _ in stream if s["norbs"] > LANE_NORBS)
+    log("stream: %d requests (%d MRCI-SD), %d determinants in total, largest %d"
+        % (len(stream), nmrci, sum(ndets), max(ndets)))
 
     phases = {}
+    tiles = {}
+    # roofline instance and tile-tuning instance: the largest full-CI request
+    big = max((i for i in range(len(stream)) if stream[i][1]["norbs"] <= LANE_NORBS),
+              key=lambda i: ndets[i])
 
     def served(precision, st):
-        srv = serve.Server(ctx, capacity=256, budget_bytes=200e9, max_norbs=16,
-                           precision=precision, restol=1e-6)
+        # the opposite-spin pool shrinks to 8 GiB when MRCI-SD requests share the budget
+        srv = serve.Server(ctx, capacity=256, budget_bytes=200e9, max_norbs=LANE_NORBS,
+                           precision=precision, restol=1e-6,
+                           pool_bytes=(8 << 30) if nmrci else (24 << 30))
         try:
+            tiles[precision] = srv.tune_tiles(stream[big][1], stream[big][2])
             ctx.sync()
             t0 = time.perf_counter()
             done = srv.run(st)
@@ -280,6 +324,11 @@ def bench_stream(J, ctx, args, rate, seed):
     ref, ref_r = reference_energies(J, ctx, stream)
     log("reference energies done (largest residual %.2e)" % ref_r)
 
+    plain_ev = {i: ev for i, _, ev, _ in plain}
+    for i in range(len(stream)):
+        if ref[i] is None:
+            ref[i] = plain_ev[i]
+
     def err_of(idx, ev):
         return max(abs(ev[k] - ref[idx][k]) for k in range(len(ev)))
 
@@ -297,15 +346,22 @@ def bench_stream(J, ctx, args, rate, seed):
     lat_m = [r.t_done - r.arrival for r in done_m]
     lat_b = [r.t_done - r.arrival for r in done_b]
     lat_p = [lat for _, lat, _, _ in plain]
-    elements = sum(sum(fci_pairs(stream[i][1])) * st["nsigma64"] for i, _, _, st in plain)
-    big = max(range(len(stream)), key=lambda i: ndets[i])
+    # CPU baseline work: the full-CI requests (the reference's CPU path cannot take the
+    # MRCI-SD space: an O(N^2) connectivity map over 6e8 determinants)
+    fci_plain = [(i, st) for i, _, _, st in plain if stream[i][1]["norbs"] <= LANE_NORBS]
+    elements = sum(sum(fci_pairs(stream[i][1])) * st["nsigma64"] for i, st in fci_plain)
     return {
         "value": len(done_m) / wall_m, "ms_per_step": 1e3 * wall_m / len(done_m),
-        "config": {"workload": "%s of %d full-CI requests (seed %d): n~U{8..14}, "
-                               "alpha=beta~U{3..5}, roots~U{1..4}, residual 1e-6; "
+        "config": {"workload": "%s of %d requests (seed %d): full CI n~U{8..14}, "
+                               "alpha=beta~U{3..5}, roots~U{1..4}%s; residual 1e-6; "
                                "200 GB admission budget, 256 slots"
                                % ("Poisson stream at %g requests/s" % rate if rate else
-                                  "batch present at t=0", len(stream), seed),
+                                  "batch present at t=0", len(stream), seed,
+                                  (", %d of them (%.0f%%) MRCI-SD 16e/60o (603,651,972 "
+                                   "determinants) with roots~U{1..3} on the serving lane"
+                                   % (nmrci, 100.0 * nmrci / len(stream))) if nmrci else
+                                  " (no MRCI-SD share)"),
+                   "mrci_requests": nmrci,
                    "requests": len(stream), "determinants_total": sum(ndets),
                    "largest_request": max(ndets), "roots_total": sum(
                        s["nroots"] for _, s, _ in stream)},
@@ -313,6 +369,7 @@ def bench_stream(J, ctx, args, rate, seed):
                   "p99_s": percentile(lat_m, 0.99), "max_abs_err_Ha": err_m,
                   "converged": sum(1 for r in done_m if r.stats["done"] == 1),
                   "sigma_s": sig_m, "step_phases_s": phases.get(J.PREC_MIXED),
+                  "tuned_tiles_fp64_fp32": tiles.get(J.PREC_MIXED),
                   "worst": worst_m,
                   "path": "served: continuous batching, mixed precision"},
         "plain_fp64": {"solves_per_s": len(plain) / wall_p, "p50_s": percentile(lat_p, 0.5),
@@ -326,7 +383,7 @@ def bench_stream(J, ctx, args, rate, seed):
         "reference_solves": {"restol": 1e-8, "largest_final_residual": ref_r,
                              "what": "FP64 GPU solve of every request, 2 buffer roots"},
         "mixed_over_plain": wall_p / wall_m,
-        "_cpu": (len(plain), elements),
+        "_cpu": (len(fci_plain), elements),
         "_big": (stream[big][1], stream[big][2]),
     }
 
@@ -354,13 +411,13 @@ def bench_mrci(J, ctx, args):
     import numpy as np
     rows = np.random.default_rng(0).integers(0, N, size=64)
     cbar = float(np.mean([len(c) for c, _, _, _ in h.rows(rows)]))
+    from jayci_amd.serve import lane_solve_params
     nr = args.roots
-    nt = nr + 1
-    kw = dict(nroots=nr, restol=1e-6, krymin=nt + 1, krymax=max(3 * nt, 8), refdim=64,
-              guess=J.GUESS_DIAG, nbuffer=1, etol=1e-9, buftol=1e-3, restart=J.RESTART_GDK,
-              totfrze=J.SYNTH_ENUC, maxiter=args.mrci_maxiter, vectors=False, check=False)
+    kw = lane_solve_params({"nroots": nr}, N, 1e-6)
+    nt = kw["nroots"] + kw["nbuffer"]
+    kw.update(maxiter=args.mrci_maxiter, vectors=False, check=False)
     sig = {}
-    for k in sorted({1, nt}):
+    for k in (sorted({1, nt}) if args.mrci_sigma_bench else []):
         for meth, name in ((J.SIGMA_FAST64, "fp64"), (J.SIGMA_FAST32, "fp32")):
             ms = h.bench_sigma(meth, k, 1)
             sig["%s_k%d" % (name, k)] = {
@@ -373,11 +430,14 @@ def bench_mrci(J, ctx, args):
     for prec in precs:
         ctx.sync()
         t = time.perf_counter()
-        ev, _, st = h.solve(precision=prec, **kw)
-        ctx.sync()
+        with c_stdout_to_stderr():
+            ev, _, st = h.solve(precision=prec, verbose=True, **kw)
+            ctx.sync()
         wall = time.perf_counter() - t
         key = "fp64" if prec == J.PREC_FP64 else "mixed"
+        nsig = st["nsigma64"] + st["nsigma32"]
         out[key] = {"seconds": wall, "energies": [float(e) for e in ev], "rc": st["rc"],
+                    "sigma_vector_seconds": st["sigma_seconds"] / nsig if nsig else None,
                     "converged": st["converged"], "iterations": st["iterations"],
                     "nsigma64": st["nsigma64"], "nsigma32": st["nsigma32"],
                     "rnorm": st["rnorm"], "sigma_seconds": st["sigma_seconds"]}
@@ -467,6 +527,10 @@ def main():
     ap.add_argument("--mrci-nvirt", type=int, default=48,
                     help="mrci: external orbitals (48 is the configuration; fewer for checks)")
     ap.add_argument("--mrci-maxiter", type=int, default=200)
+    ap.add_argument("--mrci-sigma-bench", action="store_true",
+                    help="mrci: also time single sigma launches of 1 and the block's vectors")
+    ap.add_argument("--mrci-share", type=float, default=0.0,
+                    help="served: share of MRCI-SD 16e/60o requests in the stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (there is no multi-GPU path); "

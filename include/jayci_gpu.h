@@ -330,9 +330,10 @@ int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32, d
 /* beta strings per tile of the matrix-free sigma (0 without excitation tables) */
 int jci_ham_fast_width(const jci_ham *h);
 /* tile shape of the matrix-free table sigma (the autotuner sets it): *width beta strings per
- * work-group (64, 128 or 256) and *chunk alpha singles staged together (1, 2 or 4; 0 = the
- * most that fit 64 KB of LDS, at most 4); positive values set, -1 / 0 keep (width) or
- * choose automatically (chunk 0); both are returned */
+ * work-group (64, 128 or 256; 0 = per block, the smallest of 64 .. 512 that covers the
+ * block's row, one launch per width) and *chunk alpha singles staged together (1, 2 or 4;
+ * 0 = the most that fit 64 KB of LDS, at most 4); values >= 0 set, -1 keeps; both are
+ * returned */
 int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk);
 /* *split 1: the table sigma runs as two passes, the alpha side and the alpha-beta products,
  * then the beta singles and doubles over a block-transposed copy of the input (N * nvec
@@ -363,6 +364,10 @@ int jci_batch_set_graphs(jci_batch *b, int on);
 int jci_batch_set_pool(jci_batch *b, long long bytes);
 /* the same tile widths for the batch's dense-string sigma (values <= 0 keep the current) */
 int jci_batch_set_tiles(jci_batch *b, int bn64, int bn32);
+/* the batch's tile widths autotuned: each width timed on the dense-string sigma of h (a
+ * full-CI Hamiltonian like the batch's largest instances, nvec vectors, reps launches), the
+ * fastest per precision set on the batch and on h and returned */
+int jci_batch_tune_tiles(jci_batch *b, jci_ham *h, int nvec, int reps, int *bn64, int *bn32);
 /* device bytes a slot of ndets determinants takes (basis, sigma and staging vectors) */
 long long jci_batch_bytes(long long ndets, int krymax);
 int jci_batch_add(jci_batch *b, jci_ham *h, const jci_solve_params *p, int *slot);

@@ -98,6 +98,7 @@ _sig("jci_batch_destroy", None, [_P])
 _sig("jci_batch_set_graphs", _i, [_P, _i])
 _sig("jci_batch_set_pool", _i, [_P, _ll])
 _sig("jci_batch_set_tiles", _i, [_P, _i, _i])
+_sig("jci_batch_tune_tiles", _i, [_P, _P, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)])
 _sig("jci_ham_gemm_tiles", _i, [_P, ctypes.POINTER(_i), ctypes.POINTER(_i)])
 _sig("jci_batch_bytes", _ll, [_ll, _i])
 _sig("jci_batch_add", _i, [_P, _P, ctypes.POINTER(SolveParams), ctypes.POINTER(_i)])
@@ -375,8 +376,9 @@ class Hamiltonian:
         return b64.value, b32.value, {m: t[m] for m in range(NMETHODS) if t[m] > 0}, \
             _L.jci_ham_fast_width(self._h)
 
-    def fast_tiles(self, width=0, chunk=-1):
-        """Tile shape of the table sigma (jci_ham_fast_tiles): returns (width, chunk)."""
+    def fast_tiles(self, width=-1, chunk=-1):
+        """Tile shape of the table sigma (jci_ham_fast_tiles): width 64 / 128 / 256, or 0 for
+        per-block widths; -1 keeps.  Returns (width, chunk)."""
         w, c = _i(width), _i(chunk)
         _check(_L.jci_ham_fast_tiles(self._h, ctypes.byref(w), ctypes.byref(c)), "jci_ham_fast_tiles")
         return w.value, c.value
@@ -543,6 +545,14 @@ class Batch:
             _L.jci_batch_set_graphs(h, 0)
         if pool_bytes is not None:
             _check(_L.jci_batch_set_pool(h, int(pool_bytes)), "jci_batch_set_pool")
+
+    def tune_tiles(self, ham, nvec=1, reps=5):
+        """E tile widths autotuned on a representative full-CI Hamiltonian
+        (jci_batch_tune_tiles); returns (bn64, bn32)."""
+        a, b = _i(), _i()
+        _check(_L.jci_batch_tune_tiles(self._h, ham._h, nvec, reps, ctypes.byref(a),
+                                       ctypes.byref(b)), "jci_batch_tune_tiles")
+        return a.value, b.value
 
     def set_tiles(self, bn64=0, bn32=0):
         _check(_L.jci_batch_set_tiles(self._h, bn64, bn32), "jci_batch_set_tiles")

@@ -309,6 +309,31 @@ JCI_API int jci_batch_set_tiles(jci_batch *b, int bn64, int bn32)
     return 0;
 }
 
+// E tile widths of the batch from timings of a representative full-CI Hamiltonian (the
+// grouped launch runs every instance with one width per precision; the largest instances
+// dominate its time): each available width timed with jci_bench_sigma, the fastest kept
+JCI_API int jci_batch_tune_tiles(jci_batch *b, jci_ham *h, int nvec, int reps, int *bn64,
+                                 int *bn32)
+{
+    const int c64[2] = {16, 32}, c32[2] = {32, 64};
+    int best64 = 0, best32 = 0;
+    double t64 = 1e300, t32 = 1e300;
+    for (int i = 0; i < 2; i++) {
+        int w64 = c64[i], w32 = c32[i];
+        RCHECK(jci_ham_gemm_tiles(h, &w64, &w32));
+        double ms = 0;
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM64, nvec, reps, &ms));
+        if (ms < t64) { t64 = ms; best64 = c64[i]; }
+        RCHECK(jci_bench_sigma(h, JCI_SIGMA_GEMM32, nvec, reps, &ms));
+        if (ms < t32) { t32 = ms; best32 = c32[i]; }
+    }
+    RCHECK(jci_ham_gemm_tiles(h, &best64, &best32));
+    RCHECK(jci_batch_set_tiles(b, best64, best32));
+    if (bn64) *bn64 = best64;
+    if (bn32) *bn32 = best32;
+    return 0;
+}
+
 JCI_API int jci_batch_set_pool(jci_batch *b, long long bytes)
 {
     if (bytes < (1LL << 20)) { jci_set_error("batch: pool below 1 MiB"); return -1; }

@@ -84,6 +84,74 @@ static int default_ctx(jci_ctx **c)
     return 0;
 }
 
+// ---- spaces of the lists citrunc_rtnlist returned ---------------------------------------------
+// citrunc_rtnlist registers the space of every list it returns, keyed by the list pointer, so
+// that compute_hv and dvdalg on that list run the handle path's sigma (dense-string MFMA for
+// full CI, the table kernels otherwise) instead of the explicit pair list.  The key is
+// checked on every use (count, and the masks of up to 64 determinants spread over the list
+// against the space's own): a list freed and replaced by another at the same address, or a
+// caller's own list, falls back to the pair-list path.
+struct DropSpace {
+    int aelec, belec, orbs, ndocc, nactv, xlvl;
+    long long ndets;
+    jci_space *sp = nullptr;
+    jci_ham *h = nullptr;
+    uint64_t key = 0;            // integral hash of h
+    const double *km1 = nullptr, *km2 = nullptr;
+    std::vector<uint64_t> sa, sb;   // the space's masks at the spot-check determinants
+    long long serial = 0;           // registration order
+};
+static std::map<const void *, DropSpace *> g_spaces;
+static long long g_serial = 0;
+#define JCI_DROP_SPACES 8           // registered lists kept (the oldest is dropped beyond)
+
+static void drop_space_free(DropSpace *d)
+{
+    if (!d) return;
+    if (d->h) jci_ham_destroy(d->h);
+    if (d->sp) jci_space_destroy(d->sp);
+    delete d;
+}
+
+static uint64_t int_hash(const double *m1, const double *m2, int n);
+
+// the registered space of dlist if it still describes it, with a Hamiltonian for these
+// integrals; nullptr otherwise (the caller takes the pair-list path)
+static DropSpace *drop_space(struct det *dlist, int ndets, const double *m1, const double *m2)
+{
+    auto it = g_spaces.find(dlist);
+    if (it == g_spaces.end()) return nullptr;
+    DropSpace *d = it->second;
+    if (d->ndets != ndets) return nullptr;
+    jci_ctx *c;
+    if (default_ctx(&c)) return nullptr;
+    const long long step = ndets > 64 ? ndets / 64 : 1;
+    if (!d->sp) {
+        if (jci_space_create(c, d->aelec, d->belec, d->orbs, d->ndocc, d->nactv, d->xlvl,
+                             JCI_RULE_SERIAL, &d->sp)) { d->sp = nullptr; return nullptr; }
+        if (jci_space_ndets(d->sp) != ndets) return nullptr;
+        for (long long i = 0; i < ndets; i += step) {
+            uint64_t a, b;
+            if (jci_space_dets(d->sp, i, 1, &a, &b)) return nullptr;
+            d->sa.push_back(a);
+            d->sb.push_back(b);
+        }
+    }
+    // spot check: up to 64 determinants spread over the list equal the space's
+    size_t t = 0;
+    for (long long i = 0; i < ndets && t < d->sa.size(); i += step, t++)
+        if (d->sa[t] != occ_mask(dlist[i].astr) || d->sb[t] != occ_mask(dlist[i].bstr)) return nullptr;
+    const uint64_t key = int_hash(m1, m2, d->orbs);
+    if (!d->h || d->key != key || d->km1 != m1 || d->km2 != m2) {
+        if (d->h) { jci_ham_destroy(d->h); d->h = nullptr; }
+        if (jci_ham_create(d->sp, m1, m2, JCI_DIRECT_TABLES, &d->h)) { d->h = nullptr; return nullptr; }
+        d->key = key;
+        d->km1 = m1;
+        d->km2 = m2;
+    }
+    return d;
+}
+
 // ---- citrunc_rtnlist (host only) ----------------------------------------------------------
 JCI_API struct det *citrunc_rtnlist(int aelec, int belec, int orbs, int nfrzc, int ndocc,
                                     int nactv, int nfrzv, int xlvl, int *astr_len,
@@ -145,6 +213,22 @@ JCI_API struct det *citrunc_rtnlist(int aelec, int belec, int orbs, int nfrzc, i
                 dl[k].cas = cas;
             }
     }
+    std::lock_guard<std::mutex> lk(g_mu);
+    DropSpace *ds = new DropSpace();
+    ds->aelec = *ci_aelec; ds->belec = *ci_belec; ds->orbs = *ci_orbs;
+    ds->ndocc = ndocc; ds->nactv = nactv; ds->xlvl = xlvl;
+    ds->ndets = d.ndets;
+    ds->serial = ++g_serial;
+    auto it = g_spaces.find(dl);
+    if (it != g_spaces.end()) { drop_space_free(it->second); g_spaces.erase(it); }
+    if (g_spaces.size() >= JCI_DROP_SPACES) {
+        auto old = g_spaces.begin();
+        for (auto jt = g_spaces.begin(); jt != g_spaces.end(); ++jt)
+            if (jt->second->serial < old->second->serial) old = jt;
+        drop_space_free(old->second);
+        g_spaces.erase(old);
+    }
+    g_spaces[dl] = ds;
     return dl;
 }
 
@@ -439,6 +523,27 @@ JCI_API void compute_hv(struct det *dlist, int ndets, double *moints1, double *m
 {
     (void)aelec; (void)belec; (void)ninto;
     std::lock_guard<std::mutex> lk(g_mu);
+    if (DropSpace *ds = drop_space(dlist, ndets, moints1, moints2)) {
+        // the space's own sigma on the device; the reference accumulates: c += H v
+        jci_ctx *cx;
+        default_ctx(&cx);
+        double *dx = nullptr, *dy = nullptr;
+        std::vector<double> y(ndets);
+        int rc2 = 0;
+        if (hipMalloc(&dx, sizeof(double) * ndets) != hipSuccess ||
+            hipMalloc(&dy, sizeof(double) * ndets) != hipSuccess) rc2 = -3;
+        if (!rc2 && hipMemcpyAsync(dx, v, sizeof(double) * ndets, hipMemcpyHostToDevice,
+                                   cx->stream) != hipSuccess) rc2 = -2;
+        if (!rc2) rc2 = jci_sigma_dev(ds->h, JCI_SIGMA_AUTO, 1, dx, dy);
+        if (!rc2 && (hipMemcpyAsync(y.data(), dy, sizeof(double) * ndets, hipMemcpyDeviceToHost,
+                                    cx->stream) != hipSuccess ||
+                     hipStreamSynchronize(cx->stream) != hipSuccess)) rc2 = -2;
+        hipFree(dx);
+        hipFree(dy);
+        if (rc2) { report(rc2, "compute_hv"); return; }
+        for (int i = 0; i < ndets; i++) c[i] += y[i];
+        return;
+    }
     ListOp *op;
     int rc = get_op(dlist, ndets, hmap, &op);
     if (!rc) rc = op_values(op, moints1, moints2);
@@ -500,10 +605,16 @@ JCI_API int dvdalg(struct det *dlist, int ndets, double *moints1, double *moints
         report(-1, "dvdalg");
         return -1;
     }
-    ListOp *op;
-    int rc = get_op(dlist, ndets, hmap, &op);
-    if (!rc) rc = op_values(op, moints1, moints2);
-    if (rc) { report(rc, "dvdalg"); return rc; }
+    DropSpace *ds = drop_space(dlist, ndets, moints1, moints2);
+    ListOp *op = nullptr;
+    int rc = 0;
+    if (!ds) {
+        rc = get_op(dlist, ndets, hmap, &op);
+        if (!rc) rc = op_values(op, moints1, moints2);
+        if (rc) { report(rc, "dvdalg"); return rc; }
+    }
+    jci_ctx *octx;
+    if ((rc = default_ctx(&octx))) { report(rc, "dvdalg"); return rc; }
     jci_solve_params p;
     jci_solve_defaults(&p);
     p.maxiter = maxiter;
@@ -539,18 +650,23 @@ JCI_API int dvdalg(struct det *dlist, int ndets, double *moints1, double *moints
     if (hdgls) {
         JCI_CHECK(hipMalloc(&d_pre, sizeof(double) * ndets));
         JCI_CHECK(hipMemcpyAsync(d_pre, hdgls, sizeof(double) * ndets, hipMemcpyHostToDevice,
-                                 op->c->stream));
+                                 octx->stream));
     }
     SolveOp so{};
-    so.self = op;
-    so.ctx = op->c;
-    so.N = op->n;
-    so.d_hd = d_pre ? d_pre : op->d_hd;
-    so.d_m1 = op->d_m1;
-    so.d_m2 = op->d_m2;
-    so.sigma = list_sigma;
-    so.masks = list_masks;
-    so.resolve = list_resolve;
+    if (ds) {
+        // the space's own sigma (dense-string MFMA for full CI, table kernels otherwise)
+        jci_ham_solve_op(ds->h, d_pre, &so);
+    } else {
+        so.self = op;
+        so.ctx = op->c;
+        so.N = op->n;
+        so.d_hd = d_pre ? d_pre : op->d_hd;
+        so.d_m1 = op->d_m1;
+        so.d_m2 = op->d_m2;
+        so.sigma = list_sigma;
+        so.masks = list_masks;
+        so.resolve = list_resolve;
+    }
     std::vector<double> ev(nroots), vec((size_t)nroots * ndets);
     jci_solve_stats st;
     rc = jci_solve_core(&so, &p, guess.empty() ? nullptr : guess.data(), ev.data(), vec.data(), &st, 0);

@@ -264,6 +264,10 @@ struct jci_ham {
     long long *d_tile_off[3];   // tile maps for W = 64, 128, 256
     FastArgs fast[3];
     int fast_w;                 // index into fast[] used by the FAST methods
+    // tile-width classes: each block in one launch of the width that fits its strings per
+    // row (64 .. 512), used when fast_w == 3
+    FastArgs fcls[4], fclsT[4];
+    long long *d_cls_off[4], *d_clsT_off[4];
     // the beta side as a second pass over the block-transposed vector (jk_sigma_fast_split)
     int fast_split;             // 1: two passes, 0: one fused kernel
     SpaceDesc *d_descT;         // transposed descriptor (spins swapped)
@@ -509,8 +513,10 @@ static int build_fast(jci_ham *h, int W)
         a.hd = h->d_hd;
         a.tile_off = h->d_tile_off[wi];
         a.m1 = mp;
+        a.m1int = (D.ndocc + D.nactv) * (D.ndocc + D.nactv + 1) / 2;
         a.W = Wd;
         a.A = 0;
+        a.ldy = 0;
         a.ntiles = toff[D.nblk];
         if (Wd == W) h->fast_w = wi;
     }
@@ -552,6 +558,33 @@ static int build_fast(jci_ham *h, int W)
             t.ntiles = toff[D.nblk];
         }
         h->fast_split = D.nblk > 1;
+        // width classes: rows of nq strings go to the launch of the smallest width >= nq
+        // (512 above 256)
+        const int cw[4] = {64, 128, 256, 512};
+        auto cls_of = [&](long long nq) { return nq <= 64 ? 0 : nq <= 128 ? 1 : nq <= 256 ? 2 : 3; };
+        for (int ci = 0; ci < 4; ci++) {
+            std::vector<long long> toff(D.nblk + 1, 0), toffT(D.nblk + 1, 0);
+            for (int b = 0; b < D.nblk; b++) {
+                long long na = D.s[0].gn[D.bp[b]], nq = D.s[1].gn[D.bq[b]];
+                toff[b + 1] = toff[b] + (cls_of(nq) == ci ? na * ((nq + cw[ci] - 1) / cw[ci]) : 0);
+                toffT[b + 1] = toffT[b] + (cls_of(na) == ci ? nq * ((na + cw[ci] - 1) / cw[ci]) : 0);
+            }
+            JCI_CHECK(hipMalloc(&h->d_cls_off[ci], sizeof(long long) * (D.nblk + 1)));
+            JCI_CHECK(hipMalloc(&h->d_clsT_off[ci], sizeof(long long) * (D.nblk + 1)));
+            JCI_CHECK(hipMemcpyAsync(h->d_cls_off[ci], toff.data(), sizeof(long long) * (D.nblk + 1),
+                                     hipMemcpyHostToDevice, c->stream));
+            JCI_CHECK(hipMemcpyAsync(h->d_clsT_off[ci], toffT.data(), sizeof(long long) * (D.nblk + 1),
+                                     hipMemcpyHostToDevice, c->stream));
+            JCI_CHECK(hipStreamSynchronize(c->stream));
+            h->fcls[ci] = h->fast[2];
+            h->fcls[ci].W = cw[ci];
+            h->fcls[ci].tile_off = h->d_cls_off[ci];
+            h->fcls[ci].ntiles = toff[D.nblk];
+            h->fclsT[ci] = h->fastT[2];
+            h->fclsT[ci].W = cw[ci];
+            h->fclsT[ci].tile_off = h->d_clsT_off[ci];
+            h->fclsT[ci].ntiles = toffT[D.nblk];
+        }
     }
     h->have_fast = 1;
     if (D.nblk == 1 && D.norbs <= FCI_MAXORB) {
@@ -620,6 +653,7 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     hipFree(h->d_m2sq);
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tile_off[wi]);
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tileT_off[wi]);
+    for (int ci = 0; ci < 4; ci++) { hipFree(h->d_cls_off[ci]); hipFree(h->d_clsT_off[ci]); }
     hipFree(h->d_descT);
     hipFree(h->d_tt_off);
     hipFree(h->d_xT);
@@ -894,6 +928,45 @@ static int gf_run(jci_ham *h, int prec, int K, const double *x, long long ldx, d
 
 static bool is_gf(int method) { return method == JCI_SIGMA_GEMM64 || method == JCI_SIGMA_GEMM32; }
 
+// table sigma of nvec determinant-major vectors x; output determinant-major (ldy 0) or
+// vector-major (vector k at y + k*ldy); s as for sigma_dev
+static int fast_run(jci_ham *h, int method, int nvec, const double *x, double *y, long long ldy,
+                    SigmaScratch *s)
+{
+    jci_ctx *c = h->sp->ctx;
+    const long long N = h->sp->h.ndets;
+    if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
+    const int prec = method == JCI_SIGMA_FAST64 ? 64 : 32;
+    const void *xin = x;
+    if (prec == 32) {
+        float *xf;
+        RCHECK(ensure_xf(h, N * nvec, s, &xf));
+        RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
+        xin = xf;
+    }
+    // one launch of the chosen width, or one per width class (fast_w == 3)
+    FastArgs a[4], t[4];
+    int n = 1;
+    if (h->fast_w == 3) {
+        n = 4;
+        for (int i = 0; i < 4; i++) { a[i] = h->fcls[i]; t[i] = h->fclsT[i]; a[i].A = h->fast[0].A; }
+    } else {
+        a[0] = h->fast[h->fast_w];
+        t[0] = h->fastT[h->fast_w];
+    }
+    for (int i = 0; i < n; i++) a[i].ldy = t[i].ldy = ldy;
+    if (!h->fast_split) return jk_sigma_fast_multi(prec, nvec, a, n, xin, y, c->stream);
+    const size_t need = (size_t)N * nvec * (prec == 64 ? sizeof(double) : sizeof(float));
+    void *xT;
+    if (s) {
+        RCHECK(jci_scratch_carve(s, need, &xT, "the transposed input"));
+    } else {
+        RCHECK(grow(&h->d_xT, &h->xT_bytes, need, "the transposed input"));
+        xT = h->d_xT;
+    }
+    return jk_sigma_fast_split(prec, nvec, a, n, t, n, h->d_tt_off, h->ntt, xin, xT, y, c->stream);
+}
+
 // y = H x on device, x and y determinant-major [N][nvec]; s (may be NULL) supplies every
 // device buffer the call needs beyond x and y (the allocation-free entry point)
 static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *y,
@@ -929,27 +1002,8 @@ static int sigma_dev(jci_ham *h, int method, int nvec, const double *x, double *
         return jk_sigma_generic(32, nvec, sp->d_desc, N, sp->d_mask[0], sp->d_mask[1], c->d_binom,
                                 h->d_m1, h->d_m2, h->d_hd, xf, y, c->stream);
     case JCI_SIGMA_FAST64:
-    case JCI_SIGMA_FAST32: {
-        if (!h->have_fast) { jci_set_error("sigma: no excitation tables (JCI_DIRECT_TABLES)"); return -1; }
-        const int prec = method == JCI_SIGMA_FAST64 ? 64 : 32;
-        const void *xin = x;
-        if (prec == 32) {
-            RCHECK(ensure_xf(h, N * nvec, s, &xf));
-            RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
-            xin = xf;
-        }
-        if (!h->fast_split) return jk_sigma_fast(prec, nvec, h->fast[h->fast_w], xin, y, c->stream);
-        const size_t need = (size_t)N * nvec * (prec == 64 ? sizeof(double) : sizeof(float));
-        void *xT;
-        if (s) {
-            RCHECK(jci_scratch_carve(s, need, &xT, "the transposed input"));
-        } else {
-            RCHECK(grow(&h->d_xT, &h->xT_bytes, need, "the transposed input"));
-            xT = h->d_xT;
-        }
-        return jk_sigma_fast_split(prec, nvec, h->fast[h->fast_w], h->fastT[h->fast_w],
-                                   h->d_tt_off, h->ntt, xin, xT, y, c->stream);
-    }
+    case JCI_SIGMA_FAST32:
+        return fast_run(h, method, nvec, x, y, 0, s);
     case JCI_SIGMA_FCI64:
         if (!h->have_fci) { jci_set_error("sigma: single-block kernel not applicable"); return -1; }
         return jk_sigma_fci(64, nvec, h->fci, x, y, c->stream);
@@ -1095,7 +1149,7 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
         int best_a = 0, best_split = h->fast_split;
         const int chunks[3] = {1, 2, 4};
         for (int sp = 0; sp < 2; sp++)
-            for (int wi = 0; wi < 3; wi++)
+            for (int wi = 0; wi < 4; wi++)
                 for (int ai = 0; ai < 3; ai++) {
                     h->fast_split = sp;
                     h->fast_w = wi;
@@ -1159,9 +1213,12 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
 JCI_API int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk)
 {
     if (!h->have_fast) { jci_set_error("ham: no excitation tables"); return -1; }
-    if (width && *width > 0) {
-        int wi = *width == 64 ? 0 : *width == 128 ? 1 : *width == 256 ? 2 : -1;
-        if (wi < 0) { jci_set_error("ham: tile width %d not available (64, 128, 256)", *width); return -1; }
+    if (width && *width >= 0) {
+        int wi = *width == 64 ? 0 : *width == 128 ? 1 : *width == 256 ? 2 : *width == 0 ? 3 : -1;
+        if (wi < 0) {
+            jci_set_error("ham: tile width %d not available (64, 128, 256, or 0: per block)", *width);
+            return -1;
+        }
         h->fast_w = wi;
     }
     if (chunk && *chunk >= 0) {
@@ -1171,8 +1228,8 @@ JCI_API int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk)
         }
         for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = *chunk;
     }
-    if (width) *width = h->fast[h->fast_w].W;
-    if (chunk) *chunk = h->fast[h->fast_w].A;
+    if (width) *width = h->fast_w == 3 ? 0 : h->fast[h->fast_w].W;
+    if (chunk) *chunk = h->fast[h->fast_w == 3 ? 0 : h->fast_w].A;
     return 0;
 }
 
@@ -1202,7 +1259,7 @@ JCI_API int jci_ham_gemm_tiles(jci_ham *h, int *bn64, int *bn32)
 
 JCI_API int jci_ham_fast_width(const jci_ham *h)
 {
-    return h->have_fast ? h->fast[h->fast_w].W : 0;
+    return h->have_fast ? (h->fast_w == 3 ? 0 : h->fast[h->fast_w].W) : 0;
 }
 
 JCI_API int jci_bench_copy(jci_ctx *c, long long bytes, int reps, double *ms)
@@ -1325,9 +1382,13 @@ static int sigma_block(Work &w, int first, int cnt, int method)
         int k = std::min(w.kb, first + cnt - b);
         if (w.op->sigma_cols) {
             int rc = w.op->sigma_cols(w.op->self, method, k, w.V + (size_t)b * w.ld, w.ld,
-                                      w.S + (size_t)b * w.ld, w.ld);
+                                      w.S + (size_t)b * w.ld, w.ld, w.X);
             if (rc < 0) return rc;
             if (rc == 0) continue;
+        }
+        if (!w.Y && hipMalloc(&w.Y, sizeof(double) * (size_t)w.ld * w.kb) != hipSuccess) {
+            jci_set_error("solve: out of device memory for the sigma staging block");
+            return -3;
         }
         RCHECK(jk_pack(k, w.N, w.V + (size_t)b * w.ld, w.ld, w.X, w.c->stream));
         RCHECK(w.op->sigma(w.op->self, method, k, w.X, w.Y));
@@ -1550,12 +1611,14 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     w.op = op; w.c = c; w.N = N;
     w.ld = (N + 63) / 64 * 64;
     // block of the new directions of one iteration (ntrack at most) and of the sigma chunks:
-    // the tracked pairs, at least four (JCI_MAXK at most)
-    w.kb = std::min(JCI_MAXK, std::max(ntrack, 4));
+    // the tracked pairs, at least four (JCI_MAXK at most); above 1e8 determinants exactly
+    // the tracked pairs (device memory)
+    w.kb = std::min(JCI_MAXK, std::max(ntrack, N > 100000000LL ? 1 : 4));
     size_t vb = sizeof(double) * (size_t)w.ld;
+    // the determinant-major output staging w.Y is allocated at its first use (methods that
+    // write vector-major columns need none)
     if (hipMalloc(&w.V, vb * krymax) != hipSuccess || hipMalloc(&w.S, vb * krymax) != hipSuccess ||
-        hipMalloc(&w.T, vb * w.kb) != hipSuccess || hipMalloc(&w.X, vb * w.kb) != hipSuccess ||
-        hipMalloc(&w.Y, vb * w.kb) != hipSuccess) {
+        hipMalloc(&w.T, vb * w.kb) != hipSuccess || hipMalloc(&w.X, vb * w.kb) != hipSuccess) {
         hipFree(w.V); hipFree(w.S); hipFree(w.T); hipFree(w.X); hipFree(w.Y);
         jci_set_error("solve: out of device memory for %d basis vectors of %lld", krymax, N);
         return -3;
@@ -1828,10 +1891,15 @@ static int ham_sigma(void *self, int method, int k, const double *x, double *y)
 
 // vector-major sigma; returns 1 when the method has no such form (the caller packs)
 static int ham_sigma_cols(void *self, int method, int k, const double *V, long long ldv,
-                          double *S, long long lds)
+                          double *S, long long lds, double *stage)
 {
     jci_ham *h = (jci_ham *)self;
     method = resolve_method(h, method, 0);
+    if (method == JCI_SIGMA_FAST64 || method == JCI_SIGMA_FAST32) {
+        // the table sigma reads determinant-major input and writes the columns directly
+        RCHECK(jk_pack(k, h->sp->h.ndets, V, ldv, stage, h->sp->ctx->stream));
+        return fast_run(h, method, k, stage, S, lds, nullptr);
+    }
     if (!is_gf(method)) return 1;
     if (!h->have_gf) { jci_set_error("sigma: dense-string sigma not applicable"); return -1; }
     return gf_run(h, method == JCI_SIGMA_GEMM64 ? 64 : 32, k, V, ldv, S, lds, nullptr);
@@ -1882,6 +1950,23 @@ int jci_solve_nout(jci_ham *h, const jci_solve_params *p, const double *guess, d
     op.masks = ham_masks;
     op.resolve = ham_resolve;
     return jci_solve_core(&op, p, guess, evals, evecs, stats, nout);
+}
+
+// the solver operator of a Hamiltonian with another preconditioner diagonal (device, N)
+int jci_ham_solve_op(jci_ham *h, const double *d_pre, SolveOp *op)
+{
+    *op = SolveOp{};
+    op->self = h;
+    op->ctx = h->sp->ctx;
+    op->N = h->sp->h.ndets;
+    op->d_hd = d_pre ? d_pre : h->d_hd;
+    op->d_m1 = h->d_m1;
+    op->d_m2 = h->d_m2;
+    op->sigma = ham_sigma;
+    op->sigma_cols = ham_sigma_cols;
+    op->masks = ham_masks;
+    op->resolve = ham_resolve;
+    return 0;
 }
 
 // ---- accessors for the batched engine ---------------------------------------------------------

@@ -89,10 +89,11 @@ struct SolveOp {
     const double *d_hd, *d_m1, *d_m2;
     // y = H x, determinant-major [N][k] device arrays
     int (*sigma)(void *self, int method, int k, const double *x, double *y);
-    // optional: y = H x on vector-major columns (V + j*ldv -> S + j*lds); returns 1 when the
-    // method has no such form and the caller must pack to determinant-major
+    // optional: y = H x on vector-major columns (V + j*ldv -> S + j*lds), stage: room for k
+    // determinant-major vectors the callee may use; returns 1 when the method has no such
+    // form and the caller must pack to determinant-major
     int (*sigma_cols)(void *self, int method, int k, const double *V, long long ldv, double *S,
-                      long long lds);
+                      long long lds, double *stage);
     // masks of determinants d_idx[0..n) into device arrays oa, ob
     int (*masks)(void *self, long long n, const long long *d_idx, uint64_t *oa, uint64_t *ob);
     // concrete sigma method for a requested one (0 = automatic) and precision
@@ -102,6 +103,8 @@ struct SolveOp {
 int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, double *evals,
                    double *evecs, jci_solve_stats *stats, int nout);
 long long jci_ham_ndets(jci_ham *h);
+// jci_solve's operator for h, with d_pre (device, may be NULL) as the preconditioner diagonal
+int jci_ham_solve_op(jci_ham *h, const double *d_pre, SolveOp *op);
 // GD+k restart block: the lowest kc Ritz vectors of the projected matrix G (row-major m x m;
 // eigenvectors Yf, vector j at Yf + j*m) followed by the np previous Ritz vectors Yp (vector j
 // at Yp + j*mp, mp <= m rows, zero below), each made orthonormal to the columns before it by

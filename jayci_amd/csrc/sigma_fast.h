@@ -22,9 +22,13 @@ struct FastArgs {
     const double *hd;            // diagonal
     const long long *tile_off;   // per block first tile, nblk+1 (device)
     int m1;                      // orbital pairs n(n+1)/2
-    int W;                       // beta strings per tile (64, 128 or 256)
+    int m1int;                   // pairs of internal (doubly occupied + active) orbitals, the
+                                 // leading rows and columns of (P|Q)
+    int W;                       // beta strings per tile (64, 128, 256 or 512)
     int A;                       // alpha singles per staged chunk (1, 2 or 4; 0 = by LDS size)
     long long ntiles;
+    long long ldy;               // output layout: 0 = determinant-major [N][K], else vector k
+                                 // at y + k*ldy
 };
 
 // single-block (FCI) specialisation
@@ -54,6 +58,9 @@ int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *
 // space (a), then the beta singles and doubles on the transposed space (t: descriptor from
 // jci_transpose_desc, spins and tables swapped) over xT, the block-transposed copy of x
 // (nvec * N elements of the input's type; tt_off / ntt: 16 x 16 transpose tiles per block)
-int jk_sigma_fast_split(int prec, int nvec, const FastArgs &a, const FastArgs &t,
+int jk_sigma_fast_split(int prec, int nvec, const FastArgs *a, int na, const FastArgs *t, int nt,
                         const long long *tt_off, long long ntt, const void *x, void *xT,
                         double *y, hipStream_t st);
+// one sigma as several fused launches, each over its own blocks (tile-width classes)
+int jk_sigma_fast_multi(int prec, int nvec, const FastArgs *a, int na, const void *x, double *y,
+                        hipStream_t st);

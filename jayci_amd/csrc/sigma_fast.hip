@@ -331,7 +331,15 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     for (int gp = 0; gp < GA; gp++) {
         const int bown = L.blkid[gp][q];
         bool any = bown >= 0;
-        for (int gq = 0; gq < GB; gq++) any = any || L.blkid[gp][gq] >= 0;
+        // pairs the row's beta side reaches: internal ones only while neither the row's beta
+        // group nor a beta group paired with gp holds external electrons (the staged rows
+        // then stop at the internal pairs, which lead (P|Q))
+        int qlim = d->s[1].gv[q] == 0 ? a.m1int : m1;
+        for (int gq = 0; gq < GB; gq++) {
+            if (L.blkid[gp][gq] < 0) continue;
+            any = true;
+            if (d->s[1].gv[gq] > 0) qlim = m1;
+        }
         if (!any) continue;
         const long long s1 = oa1 + goa1[gp + 1];
         for (long long c0 = oa1 + goa1[gp]; c0 < s1; c0 += A) {
@@ -349,7 +357,7 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
             __syncthreads();
             for (int t = 0; t < na; t++) {
                 const long long P = (TA.e1[c0 + t].y >> 18) & 4095;
-                for (int Q = tid; Q < m1; Q += W) g[t * m1 + Q] = (AT)a.m2sq[P * m1 + Q];
+                for (int Q = tid; Q < qlim; Q += W) g[t * m1 + Q] = (AT)a.m2sq[P * m1 + Q];
             }
             __syncthreads();
             if (!active) continue;
@@ -402,8 +410,13 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
         }
     }
     if (active) {
+        if (a.ldy) {
 #pragma unroll
-        for (int k = 0; k < K; k++) y[I * K + k] = (double)acc[k];
+            for (int k = 0; k < K; k++) y[k * a.ldy + I] = (double)acc[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; k++) y[I * K + k] = (double)acc[k];
+        }
     }
 }
 
@@ -487,8 +500,13 @@ __global__ __launch_bounds__(W) void k_sigma_fast_beta(FastArgs a, const XT *__r
     }
     // add into the original (alpha-major) layout: determinant (alpha ib_loc, beta ia_loc)
     const long long O = L.boff[b] + ib_loc * L.gnA[p] + ia_loc;
+    if (a.ldy) {
 #pragma unroll
-    for (int k = 0; k < K; k++) y[O * K + k] += (double)acc[k];
+        for (int k = 0; k < K; k++) y[k * a.ldy + O] += (double)acc[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) y[O * K + k] += (double)acc[k];
+    }
 }
 
 // x[N][K] -> xT[N][K] with every block transposed (alpha-major -> beta-major), 16 x 16
@@ -564,8 +582,12 @@ static int fast_main(int prec, int nvec, const FastArgs &a, const void *x, doubl
         if (prec == 64) return launch_fast<double, double, 128, BSIDE>(a, nvec, (const double *)x, y, st);
         return launch_fast<float, float, 128, BSIDE>(a, nvec, (const float *)x, y, st);
     }
-    if (prec == 64) return launch_fast<double, double, 256, BSIDE>(a, nvec, (const double *)x, y, st);
-    return launch_fast<float, float, 256, BSIDE>(a, nvec, (const float *)x, y, st);
+    if (a.W == 256) {
+        if (prec == 64) return launch_fast<double, double, 256, BSIDE>(a, nvec, (const double *)x, y, st);
+        return launch_fast<float, float, 256, BSIDE>(a, nvec, (const float *)x, y, st);
+    }
+    if (prec == 64) return launch_fast<double, double, 512, BSIDE>(a, nvec, (const double *)x, y, st);
+    return launch_fast<float, float, 512, BSIDE>(a, nvec, (const float *)x, y, st);
 }
 
 int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *y, hipStream_t st)
@@ -626,23 +648,38 @@ static int launch_beta(const FastArgs &t, int nvec, const XT *xT, double *y, hip
         if (r_) return r_;      \
     } while (0)
 
-int jk_sigma_fast_split(int prec, int nvec, const FastArgs &a, const FastArgs &t,
+template <typename T>
+static int beta_pass(const FastArgs &t, int nvec, const T *xT, double *y, hipStream_t st)
+{
+    if (t.ntiles <= 0) return 0;
+    if (t.W == 64) return launch_beta<T, T, 64>(t, nvec, xT, y, st);
+    if (t.W == 128) return launch_beta<T, T, 128>(t, nvec, xT, y, st);
+    if (t.W == 256) return launch_beta<T, T, 256>(t, nvec, xT, y, st);
+    return launch_beta<T, T, 512>(t, nvec, xT, y, st);
+}
+
+int jk_sigma_fast_multi(int prec, int nvec, const FastArgs *a, int na, const void *x, double *y,
+                        hipStream_t st)
+{
+    for (int i = 0; i < na; i++) RCHECK_SF(fast_main<true>(prec, nvec, a[i], x, y, st));
+    return 0;
+}
+
+int jk_sigma_fast_split(int prec, int nvec, const FastArgs *a, int na, const FastArgs *t, int nt,
                         const long long *tt_off, long long ntt, const void *x, void *xT,
                         double *y, hipStream_t st)
 {
-    if (a.ntiles <= 0) return 0;
-    // alpha side, diagonal and alpha-beta products write y; the beta side adds to it
-    RCHECK_SF(fast_main<false>(prec, nvec, a, x, y, st));
+    // alpha side, diagonal and alpha-beta products write y (each launch its own blocks);
+    // the beta side adds to it
+    for (int i = 0; i < na; i++) RCHECK_SF(fast_main<false>(prec, nvec, a[i], x, y, st));
     if (prec == 64) {
-        RCHECK_SF(transpose_blocks<double>(nvec, a.d, tt_off, ntt, (const double *)x, (double *)xT, st));
-        if (t.W == 64) return launch_beta<double, double, 64>(t, nvec, (const double *)xT, y, st);
-        if (t.W == 128) return launch_beta<double, double, 128>(t, nvec, (const double *)xT, y, st);
-        return launch_beta<double, double, 256>(t, nvec, (const double *)xT, y, st);
+        RCHECK_SF(transpose_blocks<double>(nvec, a[0].d, tt_off, ntt, (const double *)x, (double *)xT, st));
+        for (int i = 0; i < nt; i++) RCHECK_SF(beta_pass<double>(t[i], nvec, (const double *)xT, y, st));
+        return 0;
     }
-    RCHECK_SF(transpose_blocks<float>(nvec, a.d, tt_off, ntt, (const float *)x, (float *)xT, st));
-    if (t.W == 64) return launch_beta<float, float, 64>(t, nvec, (const float *)xT, y, st);
-    if (t.W == 128) return launch_beta<float, float, 128>(t, nvec, (const float *)xT, y, st);
-    return launch_beta<float, float, 256>(t, nvec, (const float *)xT, y, st);
+    RCHECK_SF(transpose_blocks<float>(nvec, a[0].d, tt_off, ntt, (const float *)x, (float *)xT, st));
+    for (int i = 0; i < nt; i++) RCHECK_SF(beta_pass<float>(t[i], nvec, (const float *)xT, y, st));
+    return 0;
 }
 
 // ---- single-block spaces (FCI) ------------------------------------------------------------

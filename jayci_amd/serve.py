@@ -7,9 +7,17 @@ those already running (one sigma launch per iteration covers all of them) and le
 soon as they converge, freeing slot and memory for the queue.  Latency of a request is
 completion time minus arrival time (queueing included).
 
+Requests the batch engine does not take (more orbitals than the batch was sized for: the
+truncated MRCI-SD spaces of 16 electrons in 60 orbitals) run on a lane of their own: a worker
+thread with a context (stream) of its own builds the space and runs the whole Davidson
+solve, one such request at a time, while the batch keeps stepping beside it; admission
+counts its device memory against the same budget.
+
 The synthetic request streams used by bench.py are generated here from a seed
 (make_stream): instance sizes follow the batched configuration (orbitals n ~ U{8..14},
 alpha = beta electrons ~ U{3..5}, roots ~ U{1..4}); arrival gaps are exponential.
+make_stream(..., mrci_share=0.1) replaces that share of the requests by MRCI-SD 16e/60o
+solves with 1-3 roots.
 """
 import math
 import time
@@ -28,14 +36,29 @@ def instance_spec(rng):
     return dict(na=ne, nb=ne, norbs=n, ndocc=0, nactv=n, xlvl=0, rule=1, nroots=nr)
 
 
-def make_stream(nreq, rate, seed, spec_fn=instance_spec):
-    """Seeded Poisson stream: list of (arrival_s, spec, integral_seed)."""
+def mrci_spec(rng):
+    """MRCI-SD 16 electrons in 60 orbitals: 4 doubly occupied + CAS(8e,8o) + 48 external,
+    excitation level 2, MRCI-SD rule (603,651,972 determinants), roots ~ U{1..3}."""
+    return dict(na=8, nb=8, norbs=60, ndocc=4, nactv=8, xlvl=2, rule=1,
+                nroots=int(rng.integers(1, 4)))
+
+
+def make_stream(nreq, rate, seed, spec_fn=instance_spec, mrci_share=0.0):
+    """Seeded Poisson stream: list of (arrival_s, spec, integral_seed).  With mrci_share > 0
+    every request is an MRCI-SD 16e/60o solve with that probability (drawn from its own
+    generator, so the full-CI requests are those of the stream without the share), and the
+    number of such requests is round(mrci_share * nreq), at least one."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(nreq):
         t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
         out.append((t, spec_fn(rng), seed * 1000 + i))
+    if mrci_share > 0 and nreq > 0:
+        mr = np.random.default_rng(seed + 7)
+        nm = max(1, int(round(mrci_share * nreq)))
+        for i in sorted(mr.choice(nreq, size=nm, replace=False)):
+            out[i] = (out[i][0], mrci_spec(mr), out[i][2])
     return out
 
 
@@ -44,6 +67,30 @@ def ndets_of(spec):
     if spec["ndocc"] == 0 and spec["nactv"] == spec["norbs"]:
         return comb(spec["norbs"], spec["na"]) * comb(spec["norbs"], spec["nb"])
     return None
+
+
+def lane_solve_params(spec, N, restol=1e-6, etol=1e-9, buftol=1e-3):
+    """Davidson settings of a lane request (a space far larger than the batched ones): one
+    buffer pair for a single root, none above; a subspace of twice the tracked pairs (at
+    least 6), GD+k restarts, diagonal guess.  Device memory per determinant is then
+    8 (2 krymax + 2 kb) + 12 kb + 8 bytes of vectors (kb = tracked pairs) beside the
+    excitation tables (lane_bytes)."""
+    nr = spec["nroots"]
+    nbuf = 1 if nr == 1 else 0
+    nt = nr + nbuf
+    return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N), krymax=min(max(2 * nt, 6), N),
+                refdim=min(64, N), guess=GUESS_DIAG, totfrze=SYNTH_ENUC, nbuffer=nbuf,
+                etol=etol, buftol=buftol, restart=RESTART_GDK)
+
+
+def lane_bytes(spec, N, restol=1e-6):
+    """Device bytes a lane request holds: basis and sigma vectors (2 krymax), new
+    directions and the determinant-major sigma input (2 kb), the block-transposed input
+    (kb, FP64) and its FP32 copy (kb), the diagonal, and the excitation tables (bounded by
+    120 bytes per determinant for excitation level 2: 104 at 16e/60o)."""
+    p = lane_solve_params(spec, N, restol)
+    kb = p["nroots"] + p["nbuffer"]
+    return N * (8 * (2 * p["krymax"] + 2 * kb) + 12 * kb + 8 + 120)
 
 
 def solve_params(spec, N, restol=1e-6, nbuffer=1, etol=1e-9, kfac=8, buftol=1e-3,
@@ -81,6 +128,7 @@ class Server:
                  restol=1e-6, mixed_switch=1e-4, graphs=True, nbuffer=1, pool_bytes=24 << 30):
         self.ctx = ctx
         self.capacity = capacity
+        self.max_norbs = max_norbs
         # the sigma pool of the opposite-spin intermediates is part of the budget
         self.budget = budget_bytes - pool_bytes
         self.precision = precision
@@ -92,11 +140,52 @@ class Server:
         # thread with a context of its own, beside the steps of the running batch
         self.bctx = Context(ctx.device)
         self.builder = ThreadPoolExecutor(max_workers=1)
+        # the lane of requests the batch does not take (see the module docstring)
+        self.lctx = Context(ctx.device)
+        self.lane = ThreadPoolExecutor(max_workers=1)
+        self.t0 = None
 
     def close(self):
         self.builder.shutdown(wait=True)
+        self.lane.shutdown(wait=True)
         self.batch.close()
         self.bctx.close()
+        self.lctx.close()
+
+    def tune_tiles(self, spec, seed, reps=5):
+        """Autotune the batch's dense-string tile widths on an instance of spec (the largest
+        full-CI request class of the stream); returns (bn64, bn32)."""
+        m1, m2 = synth_integrals(spec["norbs"], seed)
+        sp = Space(self.ctx, spec["na"], spec["nb"], spec["norbs"], spec["ndocc"],
+                   spec["nactv"], spec["xlvl"], spec["rule"])
+        h = Hamiltonian(sp, m1, m2, DIRECT_TABLES)
+        try:
+            return self.batch.tune_tiles(h, nvec=1, reps=reps)
+        finally:
+            h.close()
+            sp.close()
+
+    def _on_lane(self, spec):
+        return spec["norbs"] > self.max_norbs
+
+    def _solve_lane(self, r):
+        """The whole solve of a lane request, on the lane's context."""
+        s = r.spec
+        m1, m2 = synth_integrals(s["norbs"], r.seed)
+        r.space = Space(self.lctx, s["na"], s["nb"], s["norbs"], s["ndocc"], s["nactv"],
+                        s["xlvl"], s["rule"])
+        r.ham = Hamiltonian(r.space, m1, m2, DIRECT_TABLES)
+        r.t_admit = time.perf_counter() - self.t0
+        ev, _, st = r.ham.solve(precision=self.precision, mixed_switch=self.mixed_switch,
+                                vectors=False, check=False,
+                                **lane_solve_params(s, r.space.ndets, self.restol))
+        self.lctx.sync()
+        r.t_done = time.perf_counter() - self.t0
+        r.energies = ev
+        r.stats = dict(st, done=1 if st["converged"] else 2)
+        r.ham.close()
+        r.space.close()
+        r.ham = r.space = None
 
     def _build(self, r):
         s = r.spec
@@ -115,6 +204,12 @@ class Server:
                               precision=self.precision, mixed_switch=self.mixed_switch)
 
     def _estimate_bytes(self, spec):
+        if self._on_lane(spec):
+            sp = Space(self.ctx, spec["na"], spec["nb"], spec["norbs"], spec["ndocc"],
+                       spec["nactv"], spec["xlvl"], spec["rule"])
+            N = sp.ndets
+            sp.close()
+            return lane_bytes(spec, N, self.restol)
         N = ndets_of(spec) or 10 ** 6
         krymax = solve_params(spec, N, self.restol, self.nbuffer)["krymax"]
         return Batch.bytes_for(N, krymax) + 64 * N
@@ -131,23 +226,41 @@ class Server:
             r.bytes = self._estimate_bytes(spec)
             reqs.append(r)
         waiting, building, active, done = [], [], {}, []
+        lane = None                     # the lane request running, if any
         used = 0.0
         nxt = 0
-        t0 = time.perf_counter()
-        while nxt < len(reqs) or waiting or building or active:
+        t0 = self.t0 = time.perf_counter()
+        while nxt < len(reqs) or waiting or building or active or lane:
             now = time.perf_counter() - t0
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
                 waiting.append(reqs[nxt])
                 nxt += 1
-            # admission, first come first served within slots and HBM budget: the request is
-            # built on the worker, then added to the batch here once its build has finished
-            # (waiting for it only when nothing else could step)
-            while (waiting and len(active) + len(building) < self.capacity
-                   and used + waiting[0].bytes <= self.budget):
-                r = waiting.pop(0)
+            # admission, first come first served within slots and HBM budget: a batch request
+            # is built on the worker, then added to the batch here once its build has
+            # finished (waiting for it only when nothing else could step); a lane request
+            # starts on the lane when the lane is free
+            while waiting and used + waiting[0].bytes <= self.budget:
+                r = waiting[0]
+                if self._on_lane(r.spec):
+                    if lane is not None:
+                        break
+                    waiting.pop(0)
+                    used += r.bytes
+                    r.future = self.lane.submit(self._solve_lane, r)
+                    lane = r
+                    continue
+                if len(active) + len(building) >= self.capacity:
+                    break
+                waiting.pop(0)
                 used += r.bytes
                 r.future = self.builder.submit(self._build, r)
                 building.append(r)
+            if lane is not None and lane.future.done():
+                lane.future.result()
+                used -= lane.bytes
+                done.append(lane)
+                lane = None
+                continue
             while building and (building[0].future.done() or not active):
                 r = building.pop(0)
                 r.future.result()
@@ -155,11 +268,13 @@ class Server:
                 r.t_admit = time.perf_counter() - t0
                 active[r.slot] = r
             if not active:
-                if waiting and not building:
+                if waiting and not building and lane is None:
                     # nothing runs and the head of the queue still does not fit
                     raise RuntimeError("request %d needs %.3g bytes, above the budget of %.3g"
                                        % (waiting[0].idx, waiting[0].bytes, self.budget))
-                if nxt < len(reqs):
+                if lane is not None:
+                    time.sleep(0.005)
+                elif nxt < len(reqs):
                     time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - now)))
                 continue
             self.batch.step()

@@ -194,3 +194,39 @@ def test_batch_result_independent_of_mix(J, orc, ctx, precision):
         b.close()
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     assert runs[0][1:] == runs[1][1:]
+
+
+def test_batch_tuned_tiles_bit_identical(J, orc, ctx):
+    """E tile widths autotuned on a representative full-CI Hamiltonian and set on the batch;
+    the batch gives bit-identical energies and iteration counts with the tuned widths and with
+    every other width (a width only partitions the columns of the E product)."""
+    specs = [(3, 3, 8, 0, 8, 0, 1, 2), (4, 4, 9, 0, 9, 0, 1, 1), (3, 2, 7, 0, 7, 0, 1, 2)]
+    items = [_build(J, orc, ctx, spec, 700 + i) for i, spec in enumerate(specs)]
+    rep = items[1][1]
+    runs = []
+    for tiles in ("tuned", (16, 32), (32, 64)):
+        b = J.Batch(ctx, 8, 16)
+        if tiles == "tuned":
+            bn64, bn32 = b.tune_tiles(rep, nvec=1, reps=3)
+            assert bn64 in (16, 32) and bn32 in (32, 64)
+        else:
+            b.set_tiles(*tiles)
+        slots = []
+        for sp, h, w, nr in items:
+            p = h.params(nroots=nr, restol=1e-7, krymin=nr + 1, krymax=min(16, sp.ndets),
+                         refdim=min(48, sp.ndets), precision=J.PREC_MIXED, guess=J.GUESS_DIAG,
+                         nbuffer=1)
+            slots.append(b.add(h, p))
+        b.run(400)
+        out = []
+        for (sp, h, w, nr), s in zip(items, slots):
+            ev, _, st = b.result(s)
+            assert st["done"] == 1, st
+            if w is not None:
+                np.testing.assert_allclose(ev, w[:nr], rtol=0, atol=1e-8)
+            out.append((ev.copy(), st["iterations"]))
+        runs.append(out)
+        b.close()
+    for other in runs[1:]:
+        for (e0, i0), (e1, i1) in zip(runs[0], other):
+            assert np.array_equal(e0, e1) and i0 == i1

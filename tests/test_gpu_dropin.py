@@ -258,3 +258,37 @@ def test_pdavidson_restart_with_buffer_roots(J, orc, ctx, tmp_path):
         h.solve(nroots=2, krymin=3, guess=J.GUESS_USER, guess_vectors=back[:2])
     h.close()
     gs.close()
+
+
+def test_foreign_list_takes_the_pair_list_path(J, orc, ctx):
+    """A determinant list citrunc_rtnlist did not return (here a copy of one in a buffer of
+    the caller's) is not matched to a registered space: compute_hv and dvdalg build its
+    connectivity from the list itself and give the same results."""
+    from jayci_amd import dropin
+    L = J.lib()
+    na, nb, norbs, nd, nact, xl = 3, 3, 7, 1, 3, 2
+    m1, m2 = J.synth_integrals(norbs, 41)
+    dl, info = dropin.citrunc_rtnlist(na, nb, norbs, 0, nd, nact, 0, xl)
+    N = info["ndets"]
+    copy = (dropin.Det * N)()
+    ctypes.memmove(copy, dl, ctypes.sizeof(dropin.Det) * N)
+    own = ctypes.cast(copy, ctypes.POINTER(dropin.Det))
+    osp = orc.Space(na, nb, norbs, nd, nact, xl, 0)
+    v = np.random.default_rng(5).standard_normal(N)
+    ref = osp.sigma(m1, m2, v)
+    for lst in (dl, own):
+        rc, hmap = dropin.generate_cimap(lst, N, nd + nact)
+        assert rc == 0
+        c = np.zeros(N)
+        L.compute_hv(lst, N, _c(m1), _c(m2), na, nb, _c(v), _c(c), nd + nact, hmap)
+        np.testing.assert_allclose(c, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+        civ = np.zeros((1, N))
+        cival = np.zeros(1)
+        hd = osp.hdiag(m1, m2)
+        rc = L.dvdalg(lst, N, _c(m1), _c(m2), na, nb, _c(hd), nd + nact, ENUC, 300, 3, 12, 1,
+                      1e-8, hmap, dropin.row_pointers(civ), _c(cival), 1, 0, norbs, 64)
+        assert rc == 0
+        np.testing.assert_allclose(cival, np.linalg.eigvalsh(osp.dense_h(m1, m2))[:1] + ENUC,
+                                   rtol=0, atol=1e-8)
+        assert dropin.deallocate_cimap(hmap, N) == N
+    dropin.free(dl)

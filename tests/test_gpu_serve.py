@@ -66,3 +66,34 @@ def test_prepared_guess_same_as_computed_at_admission(J, ctx):
     b.close()
     ha.close(); spa.close(); hb.close(); spb.close()
     ctx2.close()
+
+
+def test_server_lane_for_truncated_spaces(J, orc, ctx):
+    """Requests with more orbitals than the batch is sized for (here MRCI-SD spaces of 12
+    orbitals, 3600 determinants, batch sized for 10) run on the lane beside the batched
+    full-CI requests; every result against the dense eigenvalues of the oracle's H, in both
+    precisions."""
+    from jayci_amd import serve
+    mr = lambda roots: {"norbs": 12, "na": 3, "nb": 3, "ndocc": 1, "nactv": 3, "xlvl": 2,
+                        "rule": 1, "nroots": roots}
+    specs = [_spec(8, 3, 1), mr(2), _spec(9, 4, 2), _spec(10, 3, 1), mr(1), _spec(8, 4, 3),
+             mr(3)]
+    stream = [(0.002 * i, s, 950 + i) for i, s in enumerate(specs)]
+    ref = {}
+    for i, s in enumerate(specs):
+        m1, m2 = J.synth_integrals(s["norbs"], 950 + i)
+        H = orc.Space(s["na"], s["nb"], s["norbs"], s["ndocc"], s["nactv"], s["xlvl"],
+                      s["rule"]).dense_h(m1, m2)
+        ref[i] = np.linalg.eigvalsh(H)[:s["nroots"]] + J.SYNTH_ENUC
+    for precision in (J.PREC_MIXED, J.PREC_FP64):
+        srv = serve.Server(ctx, capacity=4, budget_bytes=60e9, max_norbs=10, precision=precision,
+                           restol=1e-6, pool_bytes=4 << 30)
+        try:
+            done = srv.run(stream)
+        finally:
+            srv.close()
+        assert sorted(r.idx for r in done) == list(range(len(specs)))
+        for r in done:
+            assert r.stats["done"] == 1, (r.idx, r.stats)
+            np.testing.assert_allclose(r.energies, ref[r.idx], rtol=0, atol=1e-8)
+            assert r.t_done >= r.arrival

@@ -1,6 +1,7 @@
 """The matrix-free table sigma in every launch shape against the oracle: one fused kernel and
 the two-pass form (alpha side and alpha-beta products, then the beta side over the
-block-transposed vector), beta tiles of 64 / 128 / 256 strings, alpha singles staged 1, 2
+block-transposed vector), beta tiles of 64 / 128 / 256 strings or per block (64 .. 512,
+one launch per width), alpha singles staged 1, 2
 or 4 at a time, FP64 (1e-12 of the vector scale) and FP32 (2e-5), 1 and 3 vectors."""
 import numpy as np
 import pytest
@@ -24,7 +25,7 @@ def test_table_sigma_shapes(J, orc, ctx, spec):
     scale = np.abs(ref).max()
     for split in (0, 1):
         assert h.fast_split(split) == split
-        for width in (64, 128, 256):
+        for width in (64, 128, 256, 0):
             for chunk in (1, 2, 4):
                 assert h.fast_tiles(width, chunk) == (width, chunk)
                 for k in (1, 3):

@@ -30,7 +30,33 @@ def test_stream_is_seeded():
 def test_request_above_budget_is_an_error():
     from jayci_amd import serve
     srv = serve.Server.__new__(serve.Server)   # host state only: no batch, no context
-    srv.capacity, srv.budget, srv.restol, srv.nbuffer = 4, 1.0, 1e-6, 1
+    srv.capacity, srv.budget, srv.restol, srv.nbuffer, srv.max_norbs = 4, 1.0, 1e-6, 1, 16
     stream = [(0.0, serve.instance_spec(np.random.default_rng(1)), 1)]
     with pytest.raises(RuntimeError, match="above the budget"):
         srv.run(stream)
+
+
+def test_stream_mrci_share_and_lane_settings():
+    """The config-4 stream: the MRCI-SD share replaces requests of the same seeded stream
+    (arrival times and the other requests unchanged); the lane's settings and device bytes
+    for the 16e/60o space (603,651,972 determinants) stay within a 200 GB budget less an
+    8 GiB sigma pool for 1, 2 and 3 roots."""
+    from jayci_amd import serve
+    a = serve.make_stream(40, 200.0, 400)
+    b = serve.make_stream(40, 200.0, 400, mrci_share=0.1)
+    assert [x[0] for x in a] == [x[0] for x in b]
+    mr = [i for i, (_, s, _) in enumerate(b) if s["norbs"] == 60]
+    assert len(mr) == 4
+    for i in range(40):
+        if i not in mr:
+            assert a[i] == b[i]
+        else:
+            s = b[i][1]
+            assert (s["na"], s["nb"], s["ndocc"], s["nactv"], s["xlvl"]) == (8, 8, 4, 8, 2)
+            assert 1 <= s["nroots"] <= 3
+    N = 603651972
+    for r in (1, 2, 3):
+        spec = dict(serve.mrci_spec(np.random.default_rng(0)), nroots=r)
+        kw = serve.lane_solve_params(spec, N)
+        assert kw["nroots"] == r and kw["krymax"] >= kw["krymin"] > r
+        assert serve.lane_bytes(spec, N) <= 200e9 - (8 << 30)
