@@ -273,7 +273,8 @@ def reference_energies(J, ctx, stream):
 def bench_stream(J, ctx, args, rate, seed):
     from jayci_amd import serve
     share = args.mrci_share if rate else 0.0
-    stream = serve.make_stream(args.steps, rate, seed, mrci_share=share)
+    stream = serve.make_stream(args.steps, rate, seed, mrci_share=share,
+                               mrci_nvirt=args.mrci_nvirt)
     warm = serve.make_stream(args.warmup, 0.0, seed + 1) if args.warmup else []
     ndets = []
     for _, s, _ in stream:
@@ -357,9 +358,13 @@ def bench_stream(J, ctx, args, rate, seed):
                                "200 GB admission budget, 256 slots"
                                % ("Poisson stream at %g requests/s" % rate if rate else
                                   "batch present at t=0", len(stream), seed,
-                                  (", %d of them (%.0f%%) MRCI-SD 16e/60o (603,651,972 "
-                                   "determinants) with roots~U{1..3} on the serving lane"
-                                   % (nmrci, 100.0 * nmrci / len(stream))) if nmrci else
+                                  (", %d of them (%.0f%%) MRCI-SD 16e/%do (4 docc + "
+                                   "CAS(8e,8o) + %d external, %d determinants) with "
+                                   "roots~U{1..3} on the serving lane"
+                                   % (nmrci, 100.0 * nmrci / len(stream),
+                                      12 + args.mrci_nvirt, args.mrci_nvirt,
+                                      max(n for (_, s, _), n in zip(stream, ndets)
+                                          if s["norbs"] > LANE_NORBS))) if nmrci else
                                   " (no MRCI-SD share)"),
                    "mrci_requests": nmrci,
                    "requests": len(stream), "determinants_total": sum(ndets),
@@ -413,8 +418,14 @@ def bench_mrci(J, ctx, args):
     cbar = float(np.mean([len(c) for c, _, _, _ in h.rows(rows)]))
     from jayci_amd.serve import lane_solve_params
     nr = args.roots
-    kw = lane_solve_params({"nroots": nr}, N, 1e-6)
+    # the subspace as large as the device's free memory allows (a margin of 6 GiB kept;
+    # the diagonal and the excitation tables, 128 bytes per determinant in the lane's
+    # estimate, are allocated already)
+    free_b, total_b = J.device_memory(ctx.device)
+    kw = lane_solve_params({"nroots": nr}, N, 1e-6, budget=free_b - (6 << 30) + 128 * N)
     nt = kw["nroots"] + kw["nbuffer"]
+    log("device memory: %.1f GB free of %.1f GB; krymax %d" % (free_b / 1e9, total_b / 1e9,
+                                                              kw["krymax"]))
     kw.update(maxiter=args.mrci_maxiter, vectors=False, check=False)
     sig = {}
     for k in (sorted({1, nt}) if args.mrci_sigma_bench else []):
@@ -461,6 +472,53 @@ def bench_mrci(J, ctx, args):
     return res
 
 
+def dropin_leg(J, wl, m1, m2, gold):
+    """The reference's driver sequence through the drop-in entry points on the same space
+    (source/execute_ci_calculation.c:206-237): citrunc_rtnlist -> generate_cimap (the
+    reference's section lists, from the GPU screen) -> compute_hdgls -> dvdalg (predr 1,
+    krymin 4, krymax 16, restol 1e-6; the list comes from citrunc_rtnlist, so dvdalg runs the
+    handle path's sigma) -> deallocate_cimap, each phase timed."""
+    import ctypes
+    import numpy as np
+    from jayci_amd import dropin
+    L = J.lib()
+    c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    out = {}
+    with c_stdout_to_stderr():
+        t = time.perf_counter()
+        dl, info = dropin.citrunc_rtnlist(wl["na"], wl["nb"], wl["norbs"], 0, wl["ndocc"],
+                                          wl["nactv"], 0, wl["xlvl"])
+        out["citrunc_rtnlist_s"] = time.perf_counter() - t
+        N = info["ndets"]
+        ninto = wl["ndocc"] + wl["nactv"]
+        t = time.perf_counter()
+        rc, hmap = dropin.generate_cimap(dl, N, ninto)
+        out["generate_cimap_s"] = time.perf_counter() - t
+        hd = np.zeros(N)
+        t = time.perf_counter()
+        L.compute_hdgls(dl, N, c(m1), c(m2), wl["na"], wl["nb"], c(hd), ninto)
+        out["compute_hdgls_s"] = time.perf_counter() - t
+        civ = np.zeros((1, N))
+        cival = np.zeros(1)
+        t = time.perf_counter()
+        rc = L.dvdalg(dl, N, c(m1), c(m2), wl["na"], wl["nb"], c(hd), ninto, J.SYNTH_ENUC, 200,
+                      4, 16, 1, 1e-6, hmap, dropin.row_pointers(civ), c(cival), 1, 0,
+                      wl["norbs"], 64)
+        out["dvdalg_s"] = time.perf_counter() - t
+        t = time.perf_counter()
+        dropin.deallocate_cimap(hmap, N)
+        out["deallocate_cimap_s"] = time.perf_counter() - t
+        dropin.free(dl)
+    out["rc"] = rc
+    out["energy"] = float(cival[0])
+    out["abs_err_vs_golden_Ha"] = abs(float(cival[0]) - gold[0])
+    out["path"] = ("reference entry points (libjayci_gpu drop-in): section lists built on the "
+                   "host from the GPU screen; dvdalg on the handle path's sigma")
+    log("drop-in sequence: map %.1f s, dvdalg %.3f s, |dE| %.1e"
+        % (out["generate_cimap_s"], out["dvdalg_s"], out["abs_err_vs_golden_Ha"]))
+    return out
+
+
 def bench_fci12(J, ctx, args):
     from jayci_amd.workloads import WORKLOADS
     wl = WORKLOADS["fci12"]
@@ -495,7 +553,9 @@ def bench_fci12(J, ctx, args):
     sp.close()
     spec = dict(na=wl["na"], nb=wl["nb"], norbs=wl["norbs"], ndocc=wl["ndocc"],
                 nactv=wl["nactv"], xlvl=wl["xlvl"], rule=wl["rule"])
+    drop = dropin_leg(J, wl, m1, m2, gold) if args.dropin else None
     return {
+        "dropin": drop,
         "value": args.steps / sum(lm), "ms_per_step": 1e3 * sum(lm) / args.steps,
         "config": {"workload": wl["desc"], "ndets": N,
                    "setup_s": round(setup_s, 3), "tile_width": width,
@@ -521,25 +581,29 @@ def main():
     ap.add_argument("--workload", default="served",
                     choices=["served", "batched", "fci12", "mrci"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--dropin", action="store_true",
+                    help="fci12: also time the drop-in entry points' driver sequence")
     ap.add_argument("--roots", type=int, default=3, help="mrci: roots to solve for")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
                     help="mrci: which solves to run")
     ap.add_argument("--mrci-nvirt", type=int, default=48,
-                    help="mrci: external orbitals (48 is the configuration; fewer for checks)")
+                    help="mrci, and the served stream's MRCI-SD share: external orbitals "
+                         "(48 is the configuration; fewer for checks)")
     ap.add_argument("--mrci-maxiter", type=int, default=200)
     ap.add_argument("--mrci-sigma-bench", action="store_true",
                     help="mrci: also time single sigma launches of 1 and the block's vectors")
     ap.add_argument("--mrci-share", type=float, default=0.0,
-                    help="served: share of MRCI-SD 16e/60o requests in the stream")
+                    help="served: share of MRCI-SD 16e/(12 + mrci-nvirt)o requests in the "
+                         "stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (there is no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"served": 64, "batched": 256, "fci12": 5}[args.workload]
+        args.steps = {"served": 64, "batched": 256, "fci12": 5, "mrci": 1}[args.workload]
     if args.warmup is None:
-        args.warmup = {"served": 4, "batched": 4, "fci12": 1}[args.workload]
+        args.warmup = {"served": 4, "batched": 4, "fci12": 1, "mrci": 0}[args.workload]
 
     import jayci_amd as J
     ctx = J.Context(0)
@@ -588,6 +652,8 @@ def main():
             "config": res["config"], "mixed": res["mixed"], "plain_fp64": res["plain_fp64"]}
     if "served_fp64" in res:
         line["served_fp64"] = res["served_fp64"]
+    if res.get("dropin"):
+        line["dropin"] = res["dropin"]
     line.update({"mixed_over_plain": res["mixed_over_plain"], "roofline": roof,
                  "roofline_fp32": dense[32], "roofline_pair_list": plain_roof,
                  "cpu_baseline": cpu})

@@ -186,6 +186,8 @@ typedef struct jci_ham jci_ham;
 
 const char *jci_last_error(void);
 int jci_device_count(int *n);
+/* free and total device memory in bytes (hipMemGetInfo on that device) */
+int jci_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 
 int jci_ctx_create(int device, jci_ctx **out);
 void jci_ctx_destroy(jci_ctx *ctx);
@@ -335,11 +337,18 @@ int jci_ham_fast_width(const jci_ham *h);
  * 0 = the most that fit 64 KB of LDS, at most 4); values >= 0 set, -1 keeps; both are
  * returned */
 int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk);
-/* *split 1: the table sigma runs as two passes, the alpha side and the alpha-beta products,
- * then the beta singles and doubles over a block-transposed copy of the input (N * nvec
- * more device elements); 0: one fused kernel.  -1 keeps; the value in use is returned
- * (default 1 for spaces of more than one block; the autotuner sets it) */
+/* *split 0: the table sigma runs as one fused kernel; 1..3: as two passes over the input
+ * and a block-transposed copy of it (N * nvec more device elements): one spin's singles and
+ * doubles with the alpha-beta products, then the other spin's singles and doubles.  The
+ * lanes of a tile run over the strings of one spin: beta strings for every block (1), alpha
+ * strings for the blocks whose beta rows are short (below 256 strings) and alpha columns
+ * longer (2), alpha strings for every block (3).  -1 keeps; the value in use is returned
+ * (default 2 for spaces of more than one block, else 0; the autotuner sets it) */
 int jci_ham_fast_split(jci_ham *h, int *split);
+/* *group > 1: the table sigma deals runs of that many consecutive tiles to one XCD (they
+ * read the same vector rows and share its L2; the runs go round-robin over the 8 XCDs);
+ * 0 or 1: blockIdx order.  -1 keeps; the value in use is returned */
+int jci_ham_fast_xcd(jci_ham *h, int *group);
 /* column-tile widths of the dense-string sigma's opposite-spin product (FP64: 16 or 32,
  * FP32: 32 or 64; the autotuner sets them): a positive *bn sets, then both are returned */
 int jci_ham_gemm_tiles(jci_ham *h, int *bn64, int *bn32);

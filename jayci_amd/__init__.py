@@ -56,6 +56,7 @@ def _sig(name, res, args):
 
 _sig("jci_last_error", ctypes.c_char_p, [])
 _sig("jci_device_count", _i, [ctypes.POINTER(_i)])
+_sig("jci_device_memory", _i, [_i, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)])
 _sig("jci_ctx_create", _i, [_i, ctypes.POINTER(_P)])
 _sig("jci_ctx_destroy", None, [_P])
 _sig("jci_ctx_sync", _i, [_P])
@@ -93,6 +94,7 @@ _sig("jci_ham_autotune", _i, [_P, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)
 _sig("jci_ham_fast_width", _i, [_P])
 _sig("jci_ham_fast_tiles", _i, [_P, ctypes.POINTER(_i), ctypes.POINTER(_i)])
 _sig("jci_ham_fast_split", _i, [_P, ctypes.POINTER(_i)])
+_sig("jci_ham_fast_xcd", _i, [_P, ctypes.POINTER(_i)])
 _sig("jci_batch_create", _i, [_P, _i, _i, ctypes.POINTER(_P)])
 _sig("jci_batch_destroy", None, [_P])
 _sig("jci_batch_set_graphs", _i, [_P, _i])
@@ -176,6 +178,13 @@ def jacobi(a):
     vec = np.zeros((n, n))
     _check(_L.jci_jacobi(n, _p(a), _p(ev), _p(vec)), "jci_jacobi")
     return ev, vec
+
+
+def device_memory(device=0):
+    """(free, total) device memory in bytes."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _check(_L.jci_device_memory(device, ctypes.byref(f), ctypes.byref(t)), "jci_device_memory")
+    return f.value, t.value
 
 
 def device_count():
@@ -384,9 +393,17 @@ class Hamiltonian:
         return w.value, c.value
 
     def fast_split(self, split=-1):
-        """Two-pass table sigma on (1) or off (0); -1 queries (jci_ham_fast_split)."""
+        """Table sigma as one fused kernel (0) or two passes with every block in its own
+        orientation (1), short-row blocks transposed (2) or every block transposed (3); -1
+        queries (jci_ham_fast_split)."""
         v = _i(split)
         _check(_L.jci_ham_fast_split(self._h, ctypes.byref(v)), "jci_ham_fast_split")
+        return v.value
+
+    def fast_xcd(self, group=-1):
+        """Tiles per XCD run of the table sigma (jci_ham_fast_xcd); -1 queries."""
+        v = _i(group)
+        _check(_L.jci_ham_fast_xcd(self._h, ctypes.byref(v)), "jci_ham_fast_xcd")
         return v.value
 
     def gemm_tiles(self, bn64=0, bn32=0):

@@ -66,6 +66,18 @@ JCI_API int jci_device_count(int *n)
     return 0;
 }
 
+JCI_API int jci_device_memory(int device, size_t *free_b, size_t *total_b)
+{
+    size_t f = 0, t = 0;
+    if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess) {
+        jci_set_error("device %d: no memory information", device);
+        return -2;
+    }
+    if (free_b) *free_b = f;
+    if (total_b) *total_b = t;
+    return 0;
+}
+
 JCI_API int jci_ctx_create(int device, jci_ctx **out)
 {
     *out = nullptr;
@@ -269,7 +281,15 @@ struct jci_ham {
     FastArgs fcls[4], fclsT[4];
     long long *d_cls_off[4], *d_clsT_off[4];
     // the beta side as a second pass over the block-transposed vector (jk_sigma_fast_split)
-    int fast_split;             // 1: two passes, 0: one fused kernel
+    int fast_split;             // 0: one fused kernel; two passes: 1 every block in its own
+                                // orientation, 2 blocks of short beta rows transposed,
+                                // 3 every block transposed
+    // two-pass launches in width-class mode (set_orient): main pass (one spin's side and the
+    // alpha-beta products) and complement (the other spin's side) per width class, for the
+    // blocks kept in their orientation [0, 4) and those run transposed [4, 8)
+    FastArgs fm[8], fc[8];
+    long long *d_ooff[16];
+    int fast_xg;                // consecutive tiles per XCD run (jk sigma_fast xcd_tile)
     SpaceDesc *d_descT;         // transposed descriptor (spins swapped)
     long long *d_tileT_off[3];
     FastArgs fastT[3];
@@ -439,6 +459,61 @@ static int build_gf(jci_ham *h)
 }
 
 // excitation tables + tile map of the matrix-free sigma
+// Two-pass launch lists of the width-class mode for the orientation h->fast_split asks for.
+// A block of na alpha x nq beta strings runs in its own orientation (main pass: one alpha
+// string per tile, lanes over beta strings, on x; complement: the beta side on the
+// transposed descriptor over xT) or transposed (main pass on the transposed descriptor over
+// xT, lanes over alpha strings; complement: the alpha side on the original descriptor over
+// x).  Mode 2 transposes the blocks whose beta rows are short (nq < 256) and whose alpha
+// columns are longer, so that their tiles fill; every launch is sized by its lane count.
+static int set_orient(jci_ham *h)
+{
+    jci_space *s = h->sp;
+    jci_ctx *c = s->ctx;
+    const SpaceDesc &D = s->h;
+    const int mode = h->fast_split;
+    const int cw[4] = {64, 128, 256, 512};
+    auto cls_of = [&](long long n) { return n <= 64 ? 0 : n <= 128 ? 1 : n <= 256 ? 2 : 3; };
+    auto tiles = [&](long long rows, long long lanes, int ci) {
+        return cls_of(lanes) == ci ? rows * ((lanes + cw[ci] - 1) / cw[ci]) : 0LL;
+    };
+    std::vector<long long> off(16 * (size_t)(D.nblk + 1), 0);
+    for (int b = 0; b < D.nblk; b++) {
+        const long long na = D.s[0].gn[D.bp[b]], nq = D.s[1].gn[D.bq[b]];
+        const bool tr = mode == 3 || (mode == 2 && nq < 256 && na > nq);
+        for (int ci = 0; ci < 4; ci++) {
+            const long long t[4] = {tr ? 0 : tiles(na, nq, ci),    // main, own orientation
+                                    tr ? tiles(nq, na, ci) : 0,    // main, transposed
+                                    tr ? 0 : tiles(nq, na, ci),    // complement of the first
+                                    tr ? tiles(na, nq, ci) : 0};   // complement of the second
+            for (int k = 0; k < 4; k++) {
+                long long *o = &off[(size_t)(k * 4 + ci) * (D.nblk + 1)];
+                o[b + 1] = o[b] + t[k];
+            }
+        }
+    }
+    for (int i = 0; i < 16; i++)
+        JCI_CHECK(hipMemcpyAsync(h->d_ooff[i], &off[(size_t)i * (D.nblk + 1)],
+                                 sizeof(long long) * (D.nblk + 1), hipMemcpyHostToDevice,
+                                 c->stream));
+    JCI_CHECK(hipStreamSynchronize(c->stream));
+    for (int ci = 0; ci < 4; ci++) {
+        const FastArgs *base[4] = {&h->fast[2], &h->fastT[2], &h->fastT[2], &h->fast[2]};
+        FastArgs *dst[4] = {&h->fm[ci], &h->fm[4 + ci], &h->fc[ci], &h->fc[4 + ci]};
+        for (int k = 0; k < 4; k++) {
+            FastArgs &a = *dst[k];
+            a = *base[k];
+            a.W = cw[ci];
+            a.tile_off = h->d_ooff[k * 4 + ci];
+            a.ntiles = off[(size_t)(k * 4 + ci) * (D.nblk + 1) + D.nblk];
+            a.tout = (k == 1 || k == 2);
+            a.src_t = (k == 1 || k == 2);
+            a.hd = h->d_hd;
+        }
+    }
+    return 0;
+}
+
 static int build_fast(jci_ham *h, int W)
 {
     jci_space *s = h->sp;
@@ -517,6 +592,9 @@ static int build_fast(jci_ham *h, int W)
         a.W = Wd;
         a.A = 0;
         a.ldy = 0;
+        a.xg = 0;
+        a.tout = 0;
+        a.src_t = 0;
         a.ntiles = toff[D.nblk];
         if (Wd == W) h->fast_w = wi;
     }
@@ -556,8 +634,12 @@ static int build_fast(jci_ham *h, int W)
             t.hd = nullptr;
             t.tile_off = h->d_tileT_off[wi];
             t.ntiles = toff[D.nblk];
+            t.tout = 1;
+            t.src_t = 1;
         }
-        h->fast_split = D.nblk > 1;
+        h->fast_split = D.nblk > 1 ? 2 : 0;
+        // spaces of several blocks: per-block widths (set below) by default
+        const bool multi = D.nblk > 1;
         // width classes: rows of nq strings go to the launch of the smallest width >= nq
         // (512 above 256)
         const int cw[4] = {64, 128, 256, 512};
@@ -585,6 +667,10 @@ static int build_fast(jci_ham *h, int W)
             h->fclsT[ci].tile_off = h->d_clsT_off[ci];
             h->fclsT[ci].ntiles = toffT[D.nblk];
         }
+        if (multi) h->fast_w = 3;
+        for (int i = 0; i < 16; i++)
+            JCI_CHECK(hipMalloc(&h->d_ooff[i], sizeof(long long) * (D.nblk + 1)));
+        RCHECK(set_orient(h));
     }
     h->have_fast = 1;
     if (D.nblk == 1 && D.norbs <= FCI_MAXORB) {
@@ -654,6 +740,7 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tile_off[wi]);
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tileT_off[wi]);
     for (int ci = 0; ci < 4; ci++) { hipFree(h->d_cls_off[ci]); hipFree(h->d_clsT_off[ci]); }
+    for (int i = 0; i < 16; i++) hipFree(h->d_ooff[i]);
     hipFree(h->d_descT);
     hipFree(h->d_tt_off);
     hipFree(h->d_xT);
@@ -944,17 +1031,36 @@ static int fast_run(jci_ham *h, int method, int nvec, const double *x, double *y
         RCHECK(jk_to_f32(x, xf, N * nvec, c->stream));
         xin = xf;
     }
-    // one launch of the chosen width, or one per width class (fast_w == 3)
-    FastArgs a[4], t[4];
+    // one launch of the chosen width, or one per width class (fast_w == 3); two passes: per
+    // class and orientation (set_orient)
+    // alpha chunk per class unless one is set: 1 for rows of up to 128 strings, 2 up to 256,
+    // 4 above (LDS per work-group grows with the chunk; these keep about 16 waves per CU at
+    // 60 orbitals)
+    const int cls_a[4] = {1, 1, 2, 4};
+    const int A_set = h->fast[0].A;
+    FastArgs a[8], t[8];
     int n = 1;
-    if (h->fast_w == 3) {
+    if (h->fast_w == 3 && !h->fast_split) {
         n = 4;
-        for (int i = 0; i < 4; i++) { a[i] = h->fcls[i]; t[i] = h->fclsT[i]; a[i].A = h->fast[0].A; }
+        for (int i = 0; i < 4; i++) {
+            a[i] = h->fcls[i];
+            a[i].A = A_set > 0 ? A_set : cls_a[i];
+        }
+    } else if (h->fast_w == 3) {
+        n = 8;
+        for (int i = 0; i < 8; i++) {
+            a[i] = h->fm[i];
+            a[i].A = A_set > 0 ? A_set : cls_a[i % 4];
+            t[i] = h->fc[i];
+        }
     } else {
         a[0] = h->fast[h->fast_w];
         t[0] = h->fastT[h->fast_w];
     }
-    for (int i = 0; i < n; i++) a[i].ldy = t[i].ldy = ldy;
+    for (int i = 0; i < n; i++) {
+        a[i].ldy = t[i].ldy = ldy;
+        a[i].xg = t[i].xg = h->fast_xg;
+    }
     if (!h->fast_split) return jk_sigma_fast_multi(prec, nvec, a, n, xin, y, c->stream);
     const size_t need = (size_t)N * nvec * (prec == 64 ? sizeof(double) : sizeof(float));
     void *xT;
@@ -964,7 +1070,8 @@ static int fast_run(jci_ham *h, int method, int nvec, const double *x, double *y
         RCHECK(grow(&h->d_xT, &h->xT_bytes, need, "the transposed input"));
         xT = h->d_xT;
     }
-    return jk_sigma_fast_split(prec, nvec, a, n, t, n, h->d_tt_off, h->ntt, xin, xT, y, c->stream);
+    return jk_sigma_fast_split(prec, nvec, h->sp->d_desc, a, n, t, n, h->d_tt_off, h->ntt, xin,
+                               xT, y, c->stream);
 }
 
 // y = H x on device, x and y determinant-major [N][nvec]; s (may be NULL) supplies every
@@ -1147,11 +1254,14 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
     if (h->have_fast) {
         // beta tile width x alpha chunk of the table kernel
         int best_a = 0, best_split = h->fast_split;
-        const int chunks[3] = {1, 2, 4};
-        for (int sp = 0; sp < 2; sp++)
+        const int chunks[4] = {0, 1, 2, 4};
+        for (int sp = 0; sp < 3; sp++)
             for (int wi = 0; wi < 4; wi++)
-                for (int ai = 0; ai < 3; ai++) {
-                    h->fast_split = sp;
+                for (int ai = 0; ai < 4; ai++) {
+                    if (h->fast_split != sp) {
+                        h->fast_split = sp;
+                        RCHECK(set_orient(h));
+                    }
                     h->fast_w = wi;
                     for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = chunks[ai];
                     double ms = 0;
@@ -1161,7 +1271,10 @@ JCI_API int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *b
                     }
                 }
         h->fast_w = best_w;
-        h->fast_split = best_split;
+        if (h->fast_split != best_split) {
+            h->fast_split = best_split;
+            RCHECK(set_orient(h));
+        }
         for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = best_a;
         t[JCI_SIGMA_FAST64] = best_w_ms;
         RCHECK(jci_bench_sigma(h, JCI_SIGMA_FAST32, nvec, reps, &t[JCI_SIGMA_FAST32]));
@@ -1236,8 +1349,23 @@ JCI_API int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk)
 JCI_API int jci_ham_fast_split(jci_ham *h, int *split)
 {
     if (!h->have_fast) { jci_set_error("ham: no excitation tables"); return -1; }
-    if (split && *split >= 0) h->fast_split = *split ? 1 : 0;
+    if (split && *split > 3) { jci_set_error("ham: split mode %d not 0 .. 3", *split); return -1; }
+    if (split && *split >= 0 && *split != h->fast_split) {
+        h->fast_split = *split;
+        RCHECK(set_orient(h));
+    }
     if (split) *split = h->fast_split;
+    return 0;
+}
+
+JCI_API int jci_ham_fast_xcd(jci_ham *h, int *group)
+{
+    if (!h->have_fast) { jci_set_error("ham: no excitation tables"); return -1; }
+    if (group && *group >= 0) {
+        if (*group > 4096) { jci_set_error("ham: XCD run of %d tiles too long", *group); return -1; }
+        h->fast_xg = *group;
+    }
+    if (group) *group = h->fast_xg;
     return 0;
 }
 
@@ -1749,9 +1877,11 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
         std::vector<double> rall(nt);
         for (int r = 0; r < nt; r++) rall[r] = sqrt(c->h_small[r]);
         for (int r = 0; r < nroots; r++) { rn[r] = rall[r]; rmax = std::max(rmax, rn[r]); }
-        if (p->verbose)
+        if (p->verbose) {
             fprintf(stdout, " iter %3d  m %2d  %s  E0 %.12f  max|r| %.3e\n", it, m,
                     phase32 ? "fp32" : "fp64", theta[0] + p->totfrze, rmax);
+            fflush(stdout);
+        }
         if (phase32 && (rmax < p->mixed_switch || it == p->maxiter)) {
             // hand over to FP64: keep the lowest kc Ritz vectors of the FP32 phase (the
             // basis itself is FP64) and recompute their sigma in FP64

@@ -27,6 +27,11 @@ struct FastArgs {
     int W;                       // beta strings per tile (64, 128, 256 or 512)
     int A;                       // alpha singles per staged chunk (1, 2 or 4; 0 = by LDS size)
     long long ntiles;
+    int xg;                      // tiles per XCD run (0 / 1: blockIdx order)
+    int tout;                    // 1: the descriptor is the transpose of the output's (the
+                                 // determinant of lane ib_loc, string ia_loc is y[boff +
+                                 // ib_loc * gn(p) + ia_loc]; the diagonal is read there too)
+    int src_t;                   // 1: reads the block-transposed input xT, 0: x
     long long ldy;               // output layout: 0 = determinant-major [N][K], else vector k
                                  // at y + k*ldy
 };
@@ -54,13 +59,15 @@ int jk_exc_fill(const SpaceDesc *d, int spin, long long nstr, const uint64_t *M,
 int jk_m2_square(int m1, const double *m2, double *sq, hipStream_t st);
 int jk_sigma_fast(int prec, int nvec, const FastArgs &a, const void *x, double *y,
                   hipStream_t st);
-// the same sigma in two passes: the alpha side, diagonal and alpha-beta products on the
-// space (a), then the beta singles and doubles on the transposed space (t: descriptor from
-// jci_transpose_desc, spins and tables swapped) over xT, the block-transposed copy of x
-// (nvec * N elements of the input's type; tt_off / ntt: 16 x 16 transpose tiles per block)
-int jk_sigma_fast_split(int prec, int nvec, const FastArgs *a, int na, const FastArgs *t, int nt,
-                        const long long *tt_off, long long ntt, const void *x, void *xT,
-                        double *y, hipStream_t st);
+// the same sigma in two passes: main passes (a: one spin's singles and doubles, the
+// diagonal and the alpha-beta products; they write y), then complements (t: the other spin's
+// singles and doubles; they add), each launch over its own blocks on the original descriptor
+// dD (reading x) or on the transposed one from jci_transpose_desc (spins and tables swapped,
+// reading xT, the block-transposed copy of x made here: nvec * N elements of the input's
+// type; tt_off / ntt: 16 x 16 transpose tiles per block of dD)
+int jk_sigma_fast_split(int prec, int nvec, const SpaceDesc *dD, const FastArgs *a, int na,
+                        const FastArgs *t, int nt, const long long *tt_off, long long ntt,
+                        const void *x, void *xT, double *y, hipStream_t st);
 // one sigma as several fused launches, each over its own blocks (tile-width classes)
 int jk_sigma_fast_multi(int prec, int nvec, const FastArgs *a, int na, const void *x, double *y,
                         hipStream_t st);

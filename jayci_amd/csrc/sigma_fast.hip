@@ -222,6 +222,62 @@ __device__ __forceinline__ void load_space_lds(const SpaceDesc *__restrict__ d, 
 // together, and each beta-list entry a lane loads serves all A of them (A independent
 // gathers of x per entry instead of one entry load per gather).  Singles whose target
 // group pairs with no beta group reachable from q are skipped without staging.
+// XCD-aware tile order: work-groups are dealt to the 8 XCDs round-robin by blockIdx; with
+// G > 1, runs of G consecutive tiles (the row segments of one alpha string and its
+// neighbours, which read the same vector rows) go to one XCD and share its L2, and the runs
+// are dealt round-robin so that every XCD still gets a share of every block.  The grid is
+// padded to a multiple of 8 G; tiles past the end return at once.
+__device__ __forceinline__ long long xcd_tile(long long b, int G)
+{
+    if (G <= 1) return b;
+    const long long slot = b >> 3, x = b & 7;
+    return (slot / G) * (8LL * G) + x * G + slot % G;
+}
+
+static inline unsigned xcd_grid(long long ntiles, int G)
+{
+    if (G <= 1) return (unsigned)ntiles;
+    const long long q = 8LL * G;
+    return (unsigned)((ntiles + q - 1) / q * q);
+}
+
+// acc += sum over table entries [e0, e1) (uniform over the work-group) of v[e] times the
+// vector at row e.x of the target group: x[(base + e.x * nq) * K + k].  D entries per round
+// (D scalar entry loads, then D coalesced row loads in flight) keep the loads' latencies
+// overlapped; the tail runs one at a time.
+template <typename AT, typename XT, int K>
+__device__ __forceinline__ void uniform_rows(const int2 *__restrict__ e, const double *__restrict__ v,
+                                             long long e0, long long e1, long long base,
+                                             long long nq, const XT *__restrict__ x, AT *acc)
+{
+    constexpr int D = 16 / K >= 8 ? 8 : (16 / K >= 4 ? 4 : 2);
+    long long i = e0;
+    for (; i + D <= e1; i += D) {
+        long long J[D];
+        AT c[D];
+#pragma unroll
+        for (int u = 0; u < D; u++) {
+            J[u] = base + (long long)e[i + u].x * nq;
+            c[u] = (AT)v[i + u];
+        }
+        XT xv[D][K];
+#pragma unroll
+        for (int u = 0; u < D; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++) xv[u][k] = x[J[u] * K + k];
+#pragma unroll
+        for (int u = 0; u < D; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++) acc[k] += c[u] * (AT)xv[u][k];
+    }
+    for (; i < e1; i++) {
+        const long long J = base + (long long)e[i].x * nq;
+        const AT c = (AT)v[i];
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += c * (AT)x[J * K + k];
+    }
+}
+
 __device__ __forceinline__ int tri32(int a, int b)
 {
     return a >= b ? ((a * (a + 1)) >> 1) + b : ((b * (b + 1)) >> 1) + a;
@@ -238,7 +294,8 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     const SpaceDesc *__restrict__ d = a.d;
     const int tid = threadIdx.x;
     const int m1 = a.m1;
-    const long long tile = (long long)blockIdx.x;
+    const long long tile = xcd_tile(blockIdx.x, a.xg);
+    if (tile >= a.ntiles) return;
     const int b = tile_block(a.tile_off, d->nblk, tile);
     const int p = d->bp[b], q = d->bq[b];
     const long long nq = d->s[1].gn[q];
@@ -267,9 +324,12 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     }
     __syncthreads();
 
+    // output position (and the diagonal's): I itself, or the same determinant in the
+    // transposed layout when this descriptor is the output's transpose
+    const long long O = a.tout ? d->boff[b] + ib_loc * d->s[0].gn[p] + ia_loc : I;
     AT acc[K];
     if (active) {
-        AT h = (AT)a.hd[I];
+        AT h = (AT)a.hd[O];
 #pragma unroll
         for (int k = 0; k < K; k++) acc[k] = h * (AT)x[I * K + k];
     } else {
@@ -313,12 +373,7 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
             int bk = L.blkid[gp][q];
             if (bk < 0 || !active) continue;
             const long long base = L.boff[bk] - L.gsA[gp] * nq + ib_loc;
-            for (long long e = o2 + go2[gp]; e < o2 + go2[gp + 1]; e++) {
-                AT v = (AT)TA.v2[e];
-                long long J = base + (long long)TA.e2[e].x * nq;
-#pragma unroll
-                for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
-            }
+            uniform_rows<AT, XT, K>(TA.e2, TA.v2, o2 + go2[gp], o2 + go2[gp + 1], base, nq, x, acc);
         }
     }
 
@@ -355,9 +410,29 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
                 v1[t] = (AT)TA.v1[c0 + (t < na ? t : 0)];
             }
             __syncthreads();
-            for (int t = 0; t < na; t++) {
-                const long long P = (TA.e1[c0 + t].y >> 18) & 4095;
-                for (int Q = tid; Q < qlim; Q += W) g[t * m1 + Q] = (AT)a.m2sq[P * m1 + Q];
+            {
+                // A rows, four elements per thread and row in flight per round
+                long long prow[A];
+#pragma unroll
+                for (int t = 0; t < A; t++)
+                    prow[t] = (long long)((TA.e1[c0 + (t < na ? t : 0)].y >> 18) & 4095) * m1;
+                for (int Q0 = tid; Q0 < qlim; Q0 += 4 * W) {
+                    double v[A][4];
+#pragma unroll
+                    for (int t = 0; t < A; t++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int Q = Q0 + r * W;
+                            v[t][r] = (t < na && Q < qlim) ? a.m2sq[prow[t] + Q] : 0.0;
+                        }
+#pragma unroll
+                    for (int t = 0; t < A; t++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int Q = Q0 + r * W;
+                            if (t < na && Q < qlim) g[t * m1 + Q] = (AT)v[t][r];
+                        }
+                }
             }
             __syncthreads();
             if (!active) continue;
@@ -377,33 +452,73 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
                     for (int k = 0; k < K; k++) acc[k] += v * (AT)x[J * K + k];
                 }
             }
+            // the chunk's staged rows beyond na hold stale (possibly never written) LDS:
+            // their coefficients are selected to zero, and their gathers repeat row 0's
+            // (in bounds)
             for (int gq = 0; gq < GB; gq++) {
                 const int bk2 = L.blkid[gp][gq];
                 if (bk2 < 0) continue;
                 long long base[A];
 #pragma unroll
-                for (int t = 0; t < A; t++) base[t] = L.boff[bk2] + jl[t] * L.gnB[gq] - L.gsB[gq];
-                for (long long e2 = ob1 + gob1[gq]; e2 < ob1 + gob1[gq + 1]; e2++) {
+                for (int t = 0; t < A; t++)
+                    base[t] = L.boff[bk2] + jl[t < na ? t : 0] * L.gnB[gq] - L.gsB[gq];
+                // U beta singles per round: U * A gathers in flight, and behind them the next
+                // round's U entry loads
+                constexpr int U = 24 / (A * K) >= 4 ? 4 : (24 / (A * K) >= 2 ? 2 : 1);
+                const long long e_end = ob1 + gob1[gq + 1];
+                long long e2 = ob1 + gob1[gq];
+                if (e2 + U <= e_end) {
+                    int2 ebn[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) ebn[u] = TB.e1[e2 + u];
+                    for (; e2 + U <= e_end; e2 += U) {
+                        int2 eb[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) eb[u] = ebn[u];
+                        XT xv[U][A][K];
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+#pragma unroll
+                            for (int t = 0; t < A; t++) {
+                                const XT *xp = x + (base[t] + eb[u].x) * K;
+#pragma unroll
+                                for (int k = 0; k < K; k++) xv[u][t][k] = xp[k];
+                            }
+                        if (e2 + 2 * U <= e_end) {
+#pragma unroll
+                            for (int u = 0; u < U; u++) ebn[u] = TB.e1[e2 + U + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            const int Q = (eb[u].y >> 18) & 4095;
+                            const int nb = (eb[u].y >> 12) & 1;
+#pragma unroll
+                            for (int t = 0; t < A; t++) {
+                                AT v = t < na ? g[t * m1 + Q] : (AT)0;
+                                if (ng[t] != nb) v = -v;
+#pragma unroll
+                                for (int k = 0; k < K; k++) acc[k] += v * (AT)xv[u][t][k];
+                            }
+                        }
+                    }
+                }
+                for (; e2 < e_end; e2++) {
                     const int2 eb = TB.e1[e2];
                     const int Q = (eb.y >> 18) & 4095;
                     const int nb = (eb.y >> 12) & 1;
                     XT xv[A][K];
 #pragma unroll
                     for (int t = 0; t < A; t++) {
-                        if (t < na) {
-                            const XT *xp = x + (base[t] + eb.x) * K;
+                        const XT *xp = x + (base[t] + eb.x) * K;
 #pragma unroll
-                            for (int k = 0; k < K; k++) xv[t][k] = xp[k];
-                        }
+                        for (int k = 0; k < K; k++) xv[t][k] = xp[k];
                     }
 #pragma unroll
                     for (int t = 0; t < A; t++) {
-                        if (t < na) {
-                            AT v = g[t * m1 + Q];
-                            if (ng[t] != nb) v = -v;
+                        AT v = t < na ? g[t * m1 + Q] : (AT)0;
+                        if (ng[t] != nb) v = -v;
 #pragma unroll
-                            for (int k = 0; k < K; k++) acc[k] += v * (AT)xv[t][k];
-                        }
+                        for (int k = 0; k < K; k++) acc[k] += v * (AT)xv[t][k];
                     }
                 }
             }
@@ -412,10 +527,10 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     if (active) {
         if (a.ldy) {
 #pragma unroll
-            for (int k = 0; k < K; k++) y[k * a.ldy + I] = (double)acc[k];
+            for (int k = 0; k < K; k++) y[k * a.ldy + O] = (double)acc[k];
         } else {
 #pragma unroll
-            for (int k = 0; k < K; k++) y[I * K + k] = (double)acc[k];
+            for (int k = 0; k < K; k++) y[O * K + k] = (double)acc[k];
         }
     }
 }
@@ -438,7 +553,8 @@ __global__ __launch_bounds__(W) void k_sigma_fast_beta(FastArgs a, const XT *__r
     const SpaceDesc *__restrict__ d = a.d;          // transposed descriptor
     const int tid = threadIdx.x;
     const int m1 = a.m1;
-    const long long tile = (long long)blockIdx.x;
+    const long long tile = xcd_tile(blockIdx.x, a.xg);
+    if (tile >= a.ntiles) return;
     const int b = tile_block(a.tile_off, d->nblk, tile);
     const int p = d->bp[b], q = d->bq[b];           // beta group, alpha group
     const long long nq = d->s[1].gn[q];             // alpha strings of the block
@@ -466,15 +582,11 @@ __global__ __launch_bounds__(W) void k_sigma_fast_beta(FastArgs a, const XT *__r
             int bk = L.blkid[gp][q];
             if (bk < 0) continue;
             const long long base = L.boff[bk] - L.gsA[gp] * nq + ib_loc;
-            for (long long e = o2 + go2[gp]; e < o2 + go2[gp + 1]; e++) {
-                AT v = (AT)TA.v2[e];
-                long long J = base + (long long)TA.e2[e].x * nq;
-#pragma unroll
-                for (int k = 0; k < K; k++) acc[k] += v * (AT)xT[J * K + k];
-            }
+            uniform_rows<AT, XT, K>(TA.e2, TA.v2, o2 + go2[gp], o2 + go2[gp + 1], base, nq, xT, acc);
         }
     }
-    // singles of the (beta) string: own element with the cross-spin term
+    // singles of the (beta) string: own element with the cross-spin term (its loads issued
+    // eight at a time, in flight together)
     {
         const long long o1 = TA.off1[Ia];
         const int *go1 = TA.goff1 + Ia * (GA + 1);
@@ -486,9 +598,18 @@ __global__ __launch_bounds__(W) void k_sigma_fast_beta(FastArgs a, const XT *__r
                 const int2 en = TA.e1[e];
                 const double *__restrict__ row = a.m2sq + (long long)((en.y >> 18) & 4095) * m1;
                 AT cross = (AT)0;
-                for (uint64_t z = mb; z; z &= z - 1) {
-                    int k = jci_ctz(z);
-                    cross += (AT)row[tri32(k, k)];
+                for (uint64_t z = mb; z;) {
+                    double r8[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        r8[j] = 0.0;
+                        if (z) {
+                            r8[j] = row[tri32(jci_ctz(z), jci_ctz(z))];
+                            z &= z - 1;
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) cross += (AT)r8[j];
                 }
                 AT v = (AT)TA.v1[e] + cross;
                 if (en.y & (1 << 12)) v = -v;
@@ -498,8 +619,10 @@ __global__ __launch_bounds__(W) void k_sigma_fast_beta(FastArgs a, const XT *__r
             }
         }
     }
-    // add into the original (alpha-major) layout: determinant (alpha ib_loc, beta ia_loc)
-    const long long O = L.boff[b] + ib_loc * L.gnA[p] + ia_loc;
+    // add into the output: with a transposed descriptor the original (alpha-major)
+    // layout's determinant (alpha ib_loc, beta ia_loc), else this determinant itself
+    const long long O = a.tout ? L.boff[b] + ib_loc * L.gnA[p] + ia_loc
+                               : L.boff[b] + ia_loc * nq + ib_loc;
     if (a.ldy) {
 #pragma unroll
         for (int k = 0; k < K; k++) y[k * a.ldy + O] += (double)acc[k];
@@ -543,7 +666,7 @@ template <typename AT, typename XT, int W, int A, bool BSIDE>
 static int launch_fast_a(const FastArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
 {
     size_t shm = sizeof(AT) * (size_t)A * (size_t)a.m1;
-    dim3 g((unsigned)a.ntiles), b(W);
+    dim3 g(xcd_grid(a.ntiles, a.xg)), b(W);
     switch (nvec) {
     case 1: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 1, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
     case 2: hipLaunchKernelGGL((k_sigma_fast<AT, XT, 2, W, A, BSIDE>), g, b, shm, st, a, x, y); break;
@@ -626,7 +749,7 @@ static int transpose_blocks(int nvec, const SpaceDesc *d, const long long *tt_of
 template <typename AT, typename XT, int W>
 static int launch_beta(const FastArgs &t, int nvec, const XT *xT, double *y, hipStream_t st)
 {
-    dim3 g((unsigned)t.ntiles), b(W);
+    dim3 g(xcd_grid(t.ntiles, t.xg)), b(W);
     switch (nvec) {
     case 1: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 1, W>), g, b, 0, st, t, xT, y); break;
     case 2: hipLaunchKernelGGL((k_sigma_fast_beta<AT, XT, 2, W>), g, b, 0, st, t, xT, y); break;
@@ -665,20 +788,25 @@ int jk_sigma_fast_multi(int prec, int nvec, const FastArgs *a, int na, const voi
     return 0;
 }
 
-int jk_sigma_fast_split(int prec, int nvec, const FastArgs *a, int na, const FastArgs *t, int nt,
-                        const long long *tt_off, long long ntt, const void *x, void *xT,
-                        double *y, hipStream_t st)
+int jk_sigma_fast_split(int prec, int nvec, const SpaceDesc *dD, const FastArgs *a, int na,
+                        const FastArgs *t, int nt, const long long *tt_off, long long ntt,
+                        const void *x, void *xT, double *y, hipStream_t st)
 {
-    // alpha side, diagonal and alpha-beta products write y (each launch its own blocks);
-    // the beta side adds to it
-    for (int i = 0; i < na; i++) RCHECK_SF(fast_main<false>(prec, nvec, a[i], x, y, st));
+    // the block-transposed input first; then the main passes write y (each launch its own
+    // blocks) and the complements add to it, each reading x or xT as its descriptor needs
     if (prec == 64) {
-        RCHECK_SF(transpose_blocks<double>(nvec, a[0].d, tt_off, ntt, (const double *)x, (double *)xT, st));
-        for (int i = 0; i < nt; i++) RCHECK_SF(beta_pass<double>(t[i], nvec, (const double *)xT, y, st));
+        RCHECK_SF(transpose_blocks<double>(nvec, dD, tt_off, ntt, (const double *)x, (double *)xT, st));
+        for (int i = 0; i < na; i++)
+            RCHECK_SF(fast_main<false>(prec, nvec, a[i], a[i].src_t ? xT : x, y, st));
+        for (int i = 0; i < nt; i++)
+            RCHECK_SF(beta_pass<double>(t[i], nvec, (const double *)(t[i].src_t ? xT : x), y, st));
         return 0;
     }
-    RCHECK_SF(transpose_blocks<float>(nvec, a[0].d, tt_off, ntt, (const float *)x, (float *)xT, st));
-    for (int i = 0; i < nt; i++) RCHECK_SF(beta_pass<float>(t[i], nvec, (const float *)xT, y, st));
+    RCHECK_SF(transpose_blocks<float>(nvec, dD, tt_off, ntt, (const float *)x, (float *)xT, st));
+    for (int i = 0; i < na; i++)
+        RCHECK_SF(fast_main<false>(prec, nvec, a[i], a[i].src_t ? xT : x, y, st));
+    for (int i = 0; i < nt; i++)
+        RCHECK_SF(beta_pass<float>(t[i], nvec, (const float *)(t[i].src_t ? xT : x), y, st));
     return 0;
 }
 

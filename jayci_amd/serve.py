@@ -69,28 +69,36 @@ def ndets_of(spec):
     return None
 
 
-def lane_solve_params(spec, N, restol=1e-6, etol=1e-9, buftol=1e-3):
+def lane_vector_bytes(N, krymax, kb):
+    """Device bytes of a lane solve over N determinants: basis and sigma vectors (2 krymax),
+    new directions and the determinant-major sigma input (2 kb), the block-transposed input
+    (kb, FP64) and its FP32 copy (kb), the diagonal, and the excitation tables (bounded by
+    120 bytes per determinant for excitation level 2: 104 at 16e/60o)."""
+    return N * (8 * (2 * krymax + 2 * kb) + 12 * kb + 8 + 120)
+
+
+def lane_solve_params(spec, N, restol=1e-6, etol=1e-9, buftol=1e-3, budget=None):
     """Davidson settings of a lane request (a space far larger than the batched ones): one
-    buffer pair for a single root, none above; a subspace of twice the tracked pairs (at
-    least 6), GD+k restarts, diagonal guess.  Device memory per determinant is then
-    8 (2 krymax + 2 kb) + 12 kb + 8 bytes of vectors (kb = tracked pairs) beside the
-    excitation tables (lane_bytes)."""
+    buffer pair for a single root, none above; a subspace of four times the tracked pairs (at
+    least 8), or the largest that keeps lane_vector_bytes within `budget`, but not below
+    twice the tracked pairs (at least 6); GD+k restarts, diagonal guess.  With three tracked
+    pairs the subspace of 12 converges in 71 iterations where 9 takes 99 and 6 stalls
+    (MRCI-SD 16e/20o, 3 roots), so memory goes to the subspace first."""
     nr = spec["nroots"]
     nbuf = 1 if nr == 1 else 0
     nt = nr + nbuf
-    return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N), krymax=min(max(2 * nt, 6), N),
+    kmax, kmin = max(4 * nt, 8), max(2 * nt, 6)
+    while budget is not None and kmax > kmin and lane_vector_bytes(N, kmax, nt) > budget:
+        kmax -= 1
+    return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N), krymax=min(kmax, N),
                 refdim=min(64, N), guess=GUESS_DIAG, totfrze=SYNTH_ENUC, nbuffer=nbuf,
                 etol=etol, buftol=buftol, restart=RESTART_GDK)
 
 
-def lane_bytes(spec, N, restol=1e-6):
-    """Device bytes a lane request holds: basis and sigma vectors (2 krymax), new
-    directions and the determinant-major sigma input (2 kb), the block-transposed input
-    (kb, FP64) and its FP32 copy (kb), the diagonal, and the excitation tables (bounded by
-    120 bytes per determinant for excitation level 2: 104 at 16e/60o)."""
-    p = lane_solve_params(spec, N, restol)
-    kb = p["nroots"] + p["nbuffer"]
-    return N * (8 * (2 * p["krymax"] + 2 * kb) + 12 * kb + 8 + 120)
+def lane_bytes(spec, N, restol=1e-6, budget=None):
+    """Device bytes a lane request holds with lane_solve_params(..., budget)."""
+    p = lane_solve_params(spec, N, restol, budget=budget)
+    return lane_vector_bytes(N, p["krymax"], p["nroots"] + p["nbuffer"])
 
 
 def solve_params(spec, N, restol=1e-6, nbuffer=1, etol=1e-9, kfac=8, buftol=1e-3,
@@ -178,7 +186,8 @@ class Server:
         r.t_admit = time.perf_counter() - self.t0
         ev, _, st = r.ham.solve(precision=self.precision, mixed_switch=self.mixed_switch,
                                 vectors=False, check=False,
-                                **lane_solve_params(s, r.space.ndets, self.restol))
+                                **lane_solve_params(s, r.space.ndets, self.restol,
+                                                    budget=self.budget))
         self.lctx.sync()
         r.t_done = time.perf_counter() - self.t0
         r.energies = ev
@@ -209,7 +218,7 @@ class Server:
                        spec["nactv"], spec["xlvl"], spec["rule"])
             N = sp.ndets
             sp.close()
-            return lane_bytes(spec, N, self.restol)
+            return lane_bytes(spec, N, self.restol, budget=self.budget)
         N = ndets_of(spec) or 10 ** 6
         krymax = solve_params(spec, N, self.restol, self.nbuffer)["krymax"]
         return Batch.bytes_for(N, krymax) + 64 * N

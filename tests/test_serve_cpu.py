@@ -59,4 +59,13 @@ def test_stream_mrci_share_and_lane_settings():
         spec = dict(serve.mrci_spec(np.random.default_rng(0)), nroots=r)
         kw = serve.lane_solve_params(spec, N)
         assert kw["nroots"] == r and kw["krymax"] >= kw["krymin"] > r
-        assert serve.lane_bytes(spec, N) <= 200e9 - (8 << 30)
+        nt = r + (1 if r == 1 else 0)
+        assert kw["krymax"] == max(4 * nt, 8)
+        # within a budget the subspace shrinks, to twice the tracked pairs at the least
+        bud = 200e9 - (8 << 30)
+        kb = serve.lane_solve_params(spec, N, budget=bud)
+        assert max(2 * nt, 6) <= kb["krymax"] <= kw["krymax"]
+        assert (serve.lane_bytes(spec, N, budget=bud) <= bud
+                or kb["krymax"] == max(2 * nt, 6))
+        assert serve.lane_bytes(spec, N, budget=bud) == serve.lane_vector_bytes(
+            N, kb["krymax"], nt)
