@@ -254,14 +254,18 @@ def reference_energies(J, ctx, stream):
     1e-8 Ha.  Returns (energies, largest final residual norm)."""
     ref, rmax = [], 0.0
     for arr, s, seed in stream:
-        if s["norbs"] > LANE_NORBS:
-            ref.append(None)          # MRCI-SD requests: the plain FP64 solve is the reference
-            continue
         sp, h = make_instance(J, ctx, s, seed, J.DIRECT_TABLES)
         kw = solve_kw(J, s, sp.ndets, 1e-8)
-        kw.update(nbuffer=2, krymin=min(s["nroots"] + 3, sp.ndets),
-                  krymax=min(max(8 * (s["nroots"] + 2), 24), sp.ndets, 64), maxiter=1000,
-                  buftol=0.0, restart=J.RESTART_RITZ)
+        if s["norbs"] > LANE_NORBS:
+            # MRCI-SD requests: the lane settings at residual 1e-8 with one buffer root and a
+            # subspace of six times the tracked pairs
+            nt = s["nroots"] + 1
+            kw.update(nbuffer=1, krymin=min(nt + 1, sp.ndets),
+                      krymax=min(max(6 * nt, 8), sp.ndets), maxiter=1000, buftol=0.0)
+        else:
+            kw.update(nbuffer=2, krymin=min(s["nroots"] + 3, sp.ndets),
+                      krymax=min(max(8 * (s["nroots"] + 2), 24), sp.ndets, 64), maxiter=1000,
+                      buftol=0.0, restart=J.RESTART_RITZ)
         ev, _, st = h.solve(precision=J.PREC_FP64, check=False, **kw)
         ref.append(list(ev))
         rmax = max(rmax, max(st["rnorm"]))
@@ -299,7 +303,8 @@ def bench_stream(J, ctx, args, rate, seed):
         # the opposite-spin pool shrinks to 8 GiB when MRCI-SD requests share the budget
         srv = serve.Server(ctx, capacity=256, budget_bytes=200e9, max_norbs=LANE_NORBS,
                            precision=precision, restol=1e-6,
-                           pool_bytes=(8 << 30) if nmrci else (24 << 30))
+                           pool_bytes=(8 << 30) if nmrci else (24 << 30),
+                           lane_cu_percent=args.lane_cu)
         try:
             tiles[precision] = srv.tune_tiles(stream[big][1], stream[big][2])
             ctx.sync()
@@ -422,7 +427,8 @@ def bench_mrci(J, ctx, args):
     # the diagonal and the excitation tables, 128 bytes per determinant in the lane's
     # estimate, are allocated already)
     free_b, total_b = J.device_memory(ctx.device)
-    kw = lane_solve_params({"nroots": nr}, N, 1e-6, budget=free_b - (6 << 30) + 128 * N)
+    kw = lane_solve_params({"nroots": nr}, N, 1e-6, budget=free_b - (6 << 30) + 128 * N,
+                           kfac=args.mrci_kfac)
     nt = kw["nroots"] + kw["nbuffer"]
     log("device memory: %.1f GB free of %.1f GB; krymax %d" % (free_b / 1e9, total_b / 1e9,
                                                               kw["krymax"]))
@@ -586,13 +592,20 @@ def main():
     ap.add_argument("--roots", type=int, default=3, help="mrci: roots to solve for")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"],
                     help="mrci: which solves to run")
-    ap.add_argument("--mrci-nvirt", type=int, default=48,
-                    help="mrci, and the served stream's MRCI-SD share: external orbitals "
-                         "(48 is the configuration; fewer for checks)")
+    ap.add_argument("--mrci-nvirt", type=int, default=None,
+                    help="external orbitals of the MRCI-SD space: mrci workload (default 48, "
+                         "the configuration's), served stream's share (default 8: 16e/20o, "
+                         "17,909,892 determinants, whose solves take seconds; at 48 one takes "
+                         "tens of minutes)")
     ap.add_argument("--mrci-maxiter", type=int, default=200)
+    ap.add_argument("--mrci-kfac", type=int, default=6,
+                    help="mrci: subspace of this many times the tracked pairs, as memory "
+                         "allows (twice at the least)")
     ap.add_argument("--mrci-sigma-bench", action="store_true",
                     help="mrci: also time single sigma launches of 1 and the block's vectors")
-    ap.add_argument("--mrci-share", type=float, default=0.0,
+    ap.add_argument("--lane-cu", type=int, default=100,
+                    help="served: percent of the compute units the MRCI lane's kernels use")
+    ap.add_argument("--mrci-share", type=float, default=0.1,
                     help="served: share of MRCI-SD 16e/(12 + mrci-nvirt)o requests in the "
                          "stream")
     args = ap.parse_args()
@@ -600,6 +613,8 @@ def main():
         print("bench.py: this build measures one GPU only (there is no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.mrci_nvirt is None:
+        args.mrci_nvirt = 48 if args.workload == "mrci" else 8
     if args.steps is None:
         args.steps = {"served": 64, "batched": 256, "fci12": 5, "mrci": 1}[args.workload]
     if args.warmup is None:

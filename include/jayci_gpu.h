@@ -190,6 +190,10 @@ int jci_device_count(int *n);
 int jci_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 
 int jci_ctx_create(int device, jci_ctx **out);
+/* a context whose stream runs its kernels on cu_percent % of the device's compute units
+ * (1 .. 100; spread evenly over the CU numbering), the rest left to other streams: a long
+ * solve beside a stream of short ones keeps the latter from waiting behind its work-groups */
+int jci_ctx_create_cus(int device, int cu_percent, jci_ctx **out);
 void jci_ctx_destroy(jci_ctx *ctx);
 int jci_ctx_sync(jci_ctx *ctx);
 /* the HIP stream the context's work is queued on (a hipStream_t) */
@@ -256,7 +260,9 @@ typedef struct {
     int maxiter, krymin, krymax, nroots, refdim;
     double restol;
     int precision;        /* JCI_PREC_FP64 or JCI_PREC_MIXED */
-    double mixed_switch;  /* residual norm below which FP32 sigma hands over to FP64 */
+    double mixed_switch;  /* residual norm below which FP32 sigma hands over to FP64 (it
+                           * also hands over when the largest residual has made no new 10 %
+                           * minimum for 6 iterations: the FP32 noise floor) */
     int sigma_fp64, sigma_fp32; /* JCI_SIGMA_* used in each phase (0 = automatic) */
     int guess;            /* JCI_GUESS_* */
     double totfrze;       /* added to the eigenvalues (nuclear repulsion + frozen core) */
@@ -333,7 +339,7 @@ int jci_ham_autotune(jci_ham *h, int nvec, int reps, int *best64, int *best32, d
 int jci_ham_fast_width(const jci_ham *h);
 /* tile shape of the matrix-free table sigma (the autotuner sets it): *width beta strings per
  * work-group (64, 128 or 256; 0 = per block, the smallest of 64 .. 512 that covers the
- * block's row, one launch per width) and *chunk alpha singles staged together (1, 2 or 4;
+ * block's row, one launch per width) and *chunk alpha singles staged together (1, 2, 4 or 8;
  * 0 = the most that fit 64 KB of LDS, at most 4); values >= 0 set, -1 keeps; both are
  * returned */
 int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk);
@@ -345,10 +351,13 @@ int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk);
  * longer (2), alpha strings for every block (3).  -1 keeps; the value in use is returned
  * (default 2 for spaces of more than one block, else 0; the autotuner sets it) */
 int jci_ham_fast_split(jci_ham *h, int *split);
-/* *group > 1: the table sigma deals runs of that many consecutive tiles to one XCD (they
- * read the same vector rows and share its L2; the runs go round-robin over the 8 XCDs);
- * 0 or 1: blockIdx order.  -1 keeps; the value in use is returned */
-int jci_ham_fast_xcd(jci_ham *h, int *group);
+/* tile order of the table sigma.  *group > 1: runs of that many consecutive tiles go to one
+ * XCD (they share its L2; the runs go round-robin over the 8 XCDs); 0 or 1: blockIdx order.
+ * *chunk_major 1: inside a block, consecutive tiles are consecutive strings on one lane
+ * chunk (neighbouring strings share most excitation targets, so such tiles read the same
+ * vector rows over the same columns); 0: the chunks of one string are consecutive.  -1
+ * keeps; the values in use are returned */
+int jci_ham_fast_xcd(jci_ham *h, int *group, int *chunk_major);
 /* column-tile widths of the dense-string sigma's opposite-spin product (FP64: 16 or 32,
  * FP32: 32 or 64; the autotuner sets them): a positive *bn sets, then both are returned */
 int jci_ham_gemm_tiles(jci_ham *h, int *bn64, int *bn32);

@@ -58,6 +58,7 @@ _sig("jci_last_error", ctypes.c_char_p, [])
 _sig("jci_device_count", _i, [ctypes.POINTER(_i)])
 _sig("jci_device_memory", _i, [_i, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)])
 _sig("jci_ctx_create", _i, [_i, ctypes.POINTER(_P)])
+_sig("jci_ctx_create_cus", _i, [_i, _i, ctypes.POINTER(_P)])
 _sig("jci_ctx_destroy", None, [_P])
 _sig("jci_ctx_sync", _i, [_P])
 _sig("jci_ctx_stream", _P, [_P])
@@ -94,7 +95,7 @@ _sig("jci_ham_autotune", _i, [_P, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)
 _sig("jci_ham_fast_width", _i, [_P])
 _sig("jci_ham_fast_tiles", _i, [_P, ctypes.POINTER(_i), ctypes.POINTER(_i)])
 _sig("jci_ham_fast_split", _i, [_P, ctypes.POINTER(_i)])
-_sig("jci_ham_fast_xcd", _i, [_P, ctypes.POINTER(_i)])
+_sig("jci_ham_fast_xcd", _i, [_P, ctypes.POINTER(_i), ctypes.POINTER(_i)])
 _sig("jci_batch_create", _i, [_P, _i, _i, ctypes.POINTER(_P)])
 _sig("jci_batch_destroy", None, [_P])
 _sig("jci_batch_set_graphs", _i, [_P, _i])
@@ -194,9 +195,15 @@ def device_count():
 
 
 class Context:
-    def __init__(self, device=0):
+    def __init__(self, device=0, cu_percent=100):
+        """A device context and its stream; cu_percent < 100 runs the stream's kernels on that
+        share of the compute units (jci_ctx_create_cus)."""
         h = _P()
-        _check(_L.jci_ctx_create(device, ctypes.byref(h)), "jci_ctx_create")
+        if cu_percent < 100:
+            _check(_L.jci_ctx_create_cus(device, cu_percent, ctypes.byref(h)),
+                   "jci_ctx_create_cus")
+        else:
+            _check(_L.jci_ctx_create(device, ctypes.byref(h)), "jci_ctx_create")
         self._h = h
         self.device = device
 
@@ -400,11 +407,12 @@ class Hamiltonian:
         _check(_L.jci_ham_fast_split(self._h, ctypes.byref(v)), "jci_ham_fast_split")
         return v.value
 
-    def fast_xcd(self, group=-1):
-        """Tiles per XCD run of the table sigma (jci_ham_fast_xcd); -1 queries."""
-        v = _i(group)
-        _check(_L.jci_ham_fast_xcd(self._h, ctypes.byref(v)), "jci_ham_fast_xcd")
-        return v.value
+    def fast_xcd(self, group=-1, chunk_major=-1):
+        """Tile order of the table sigma (jci_ham_fast_xcd): tiles per XCD run and chunk-major
+        order inside a block; -1 keeps.  Returns (group, chunk_major)."""
+        v, c = _i(group), _i(chunk_major)
+        _check(_L.jci_ham_fast_xcd(self._h, ctypes.byref(v), ctypes.byref(c)), "jci_ham_fast_xcd")
+        return v.value, c.value
 
     def gemm_tiles(self, bn64=0, bn32=0):
         """E tile widths of the dense-string sigma; positive arguments set them."""

@@ -34,10 +34,13 @@ def _run(cmd):
     return r.stdout
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compile every source (up to `jobs` hipcc processes at once, default the CPU count
+    capped at 4) and link the library."""
+    from concurrent.futures import ThreadPoolExecutor
     objdir = os.path.join(HERE, "_obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in HIP_SOURCES + CXX_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
@@ -47,10 +50,15 @@ def build(verbose=False):
                                            "-c", path, "-o", obj]
         if src.endswith(".cpp"):
             cmd = [HIPCC] + COMMON + DEVICE + ["-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
+        cmds.append(cmd)
         objs.append(obj)
+    jobs = jobs or max(1, min(4, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        outs = list(ex.map(_run, cmds))
+    if verbose:
+        for out in outs:
+            if out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
          ["-lamdhip64"])

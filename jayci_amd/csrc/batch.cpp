@@ -51,6 +51,8 @@ struct BInst {
     size_t mem_bytes = 0;
     int m = 0, krymin = 0, krymax = 0, nroots = 0, ntrack = 0;
     int phase32 = 0;
+    double best32 = 1e300;             // FP32 phase: smallest largest-residual so far, and when
+    int it_best32 = 0;
     int sig_first = 0, sig_count = 0;  // basis columns whose sigma is pending
     int kc = 0;                        // collapse target of this step (0: none)
     int iters = 0;
@@ -977,7 +979,9 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
             rmax = std::max(rmax, s.rn[r]);
         }
         for (int r = 0; r < s.nroots; r++) s.ev[r] = s.theta[r] + s.p.totfrze;
-        if (s.phase32 && (rmax < s.p.mixed_switch || s.iters >= s.p.maxiter)) {
+        if (s.phase32 && rmax < JCI_MIXED_GAIN * s.best32) { s.best32 = rmax; s.it_best32 = s.iters; }
+        if (s.phase32 && (rmax < s.p.mixed_switch || s.iters >= s.p.maxiter ||
+                          s.iters - s.it_best32 >= JCI_MIXED_STALL)) {
             s.phase32 = 0;
             handover.push_back(act[t]);
             continue;

@@ -78,17 +78,34 @@ JCI_API int jci_device_memory(int device, size_t *free_b, size_t *total_b)
     return 0;
 }
 
-JCI_API int jci_ctx_create(int device, jci_ctx **out)
+JCI_API int jci_ctx_create_cus(int device, int cu_percent, jci_ctx **out)
 {
     *out = nullptr;
     int n = 0;
     jci_device_count(&n);
     if (n <= 0) { jci_set_error("no HIP device visible"); return -2; }
     if (device < 0 || device >= n) { jci_set_error("device %d out of range", device); return -1; }
+    if (cu_percent < 1 || cu_percent > 100) {
+        jci_set_error("CU share %d%% not in 1 .. 100", cu_percent);
+        return -1;
+    }
     JCI_CHECK(hipSetDevice(device));
+    hipStream_t st = nullptr;
+    if (cu_percent < 100) {
+        // every CU i with floor((i + 1) p / 100) > floor(i p / 100): p % of them, spread
+        // evenly over the device's CU numbering (and so over its XCDs)
+        int ncu = 0;
+        JCI_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+        std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+        for (int i = 0; i < ncu; i++)
+            if ((i + 1) * cu_percent / 100 > i * cu_percent / 100) mask[i / 32] |= 1u << (i % 32);
+        JCI_CHECK(hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()));
+    } else {
+        JCI_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    }
     jci_ctx *c = (jci_ctx *)calloc(1, sizeof(jci_ctx));
     c->device = device;
-    JCI_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->stream = st;
     JCI_CHECK(hipMalloc(&c->d_partial, sizeof(double) * JCI_PARTIAL_BLOCKS * JCI_MAXBASIS * JCI_MAXBASIS));
     JCI_CHECK(hipMalloc(&c->d_small, sizeof(double) * JCI_MAXBASIS * JCI_MAXBASIS));
     JCI_CHECK(hipHostMalloc(&c->h_small, sizeof(double) * JCI_MAXBASIS * JCI_MAXBASIS, 0));
@@ -101,6 +118,11 @@ JCI_API int jci_ctx_create(int device, jci_ctx **out)
     if (rc) return rc;
     *out = c;
     return 0;
+}
+
+JCI_API int jci_ctx_create(int device, jci_ctx **out)
+{
+    return jci_ctx_create_cus(device, 100, out);
 }
 
 static void ctx_free(jci_ctx *c)
@@ -290,6 +312,7 @@ struct jci_ham {
     FastArgs fm[8], fc[8];
     long long *d_ooff[16];
     int fast_xg;                // consecutive tiles per XCD run (jk sigma_fast xcd_tile)
+    int fast_cm;                // chunk-major tile order in a block (sigma_fast tile_pos)
     SpaceDesc *d_descT;         // transposed descriptor (spins swapped)
     long long *d_tileT_off[3];
     FastArgs fastT[3];
@@ -638,6 +661,10 @@ static int build_fast(jci_ham *h, int W)
             t.src_t = 1;
         }
         h->fast_split = D.nblk > 1 ? 2 : 0;
+        // tiles chunk-major, in runs of 64 per XCD (16e/60o FP64 pass: 9.8 s string-major,
+        // 7.8 s chunk-major, 7.5 s chunk-major in runs of 64)
+        h->fast_cm = 1;
+        h->fast_xg = 64;
         // spaces of several blocks: per-block widths (set below) by default
         const bool multi = D.nblk > 1;
         // width classes: rows of nq strings go to the launch of the smallest width >= nq
@@ -1033,10 +1060,11 @@ static int fast_run(jci_ham *h, int method, int nvec, const double *x, double *y
     }
     // one launch of the chosen width, or one per width class (fast_w == 3); two passes: per
     // class and orientation (set_orient)
-    // alpha chunk per class unless one is set: 1 for rows of up to 128 strings, 2 up to 256,
-    // 4 above (LDS per work-group grows with the chunk; these keep about 16 waves per CU at
-    // 60 orbitals)
-    const int cls_a[4] = {1, 1, 2, 4};
+    // alpha chunk per class unless one is set: FP64 1 for rows of up to 128 strings, 2 up
+    // to 256, 4 above, FP32 twice that (LDS per work-group grows with the chunk and the
+    // element size; these keep about 16 waves per CU at 60 orbitals)
+    const int cls_a[4] = {prec == 64 ? 1 : 2, prec == 64 ? 1 : 2, prec == 64 ? 2 : 4,
+                          prec == 64 ? 4 : 8};
     const int A_set = h->fast[0].A;
     FastArgs a[8], t[8];
     int n = 1;
@@ -1060,6 +1088,7 @@ static int fast_run(jci_ham *h, int method, int nvec, const double *x, double *y
     for (int i = 0; i < n; i++) {
         a[i].ldy = t[i].ldy = ldy;
         a[i].xg = t[i].xg = h->fast_xg;
+        a[i].cm = t[i].cm = h->fast_cm;
     }
     if (!h->fast_split) return jk_sigma_fast_multi(prec, nvec, a, n, xin, y, c->stream);
     const size_t need = (size_t)N * nvec * (prec == 64 ? sizeof(double) : sizeof(float));
@@ -1335,8 +1364,8 @@ JCI_API int jci_ham_fast_tiles(jci_ham *h, int *width, int *chunk)
         h->fast_w = wi;
     }
     if (chunk && *chunk >= 0) {
-        if (*chunk != 0 && *chunk != 1 && *chunk != 2 && *chunk != 4) {
-            jci_set_error("ham: alpha chunk %d not available (0, 1, 2, 4)", *chunk);
+        if (*chunk != 0 && *chunk != 1 && *chunk != 2 && *chunk != 4 && *chunk != 8) {
+            jci_set_error("ham: alpha chunk %d not available (0, 1, 2, 4, 8)", *chunk);
             return -1;
         }
         for (int w2 = 0; w2 < 3; w2++) h->fast[w2].A = *chunk;
@@ -1358,14 +1387,16 @@ JCI_API int jci_ham_fast_split(jci_ham *h, int *split)
     return 0;
 }
 
-JCI_API int jci_ham_fast_xcd(jci_ham *h, int *group)
+JCI_API int jci_ham_fast_xcd(jci_ham *h, int *group, int *chunk_major)
 {
     if (!h->have_fast) { jci_set_error("ham: no excitation tables"); return -1; }
     if (group && *group >= 0) {
         if (*group > 4096) { jci_set_error("ham: XCD run of %d tiles too long", *group); return -1; }
         h->fast_xg = *group;
     }
+    if (chunk_major && *chunk_major >= 0) h->fast_cm = *chunk_major ? 1 : 0;
     if (group) *group = h->fast_xg;
+    if (chunk_major) *chunk_major = h->fast_cm;
     return 0;
 }
 
@@ -1754,6 +1785,8 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
     int rc = 0;
     int mixed = p->precision == JCI_PREC_MIXED;
     int phase32 = mixed;
+    double best32 = 1e300;      // FP32 phase: smallest largest-residual so far and when
+    int it_best32 = 0;
     int m64 = op->resolve(op->self, p->sigma_fp64, 0), m32 = op->resolve(op->self, p->sigma_fp32, 1);
     auto cur_method = [&]() { return phase32 ? m32 : m64; };
     auto count_sigma = [&](int n) { if (phase32) w.ns32 += n; else w.ns64 += n; };
@@ -1882,8 +1915,11 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
                     phase32 ? "fp32" : "fp64", theta[0] + p->totfrze, rmax);
             fflush(stdout);
         }
-        if (phase32 && (rmax < p->mixed_switch || it == p->maxiter)) {
-            // hand over to FP64: keep the lowest kc Ritz vectors of the FP32 phase (the
+        if (phase32 && rmax < JCI_MIXED_GAIN * best32) { best32 = rmax; it_best32 = it; }
+        if (phase32 && (rmax < p->mixed_switch || it == p->maxiter ||
+                        it - it_best32 >= JCI_MIXED_STALL)) {
+            // hand over to FP64 (below the switch residual, or stalled at the FP32 sigma's
+            // noise floor above it): keep the lowest kc Ritz vectors of the FP32 phase (the
             // basis itself is FP64) and recompute their sigma in FP64
             phase32 = 0;
             int kc = std::min(m, krymin);

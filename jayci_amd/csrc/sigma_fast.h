@@ -25,9 +25,10 @@ struct FastArgs {
     int m1int;                   // pairs of internal (doubly occupied + active) orbitals, the
                                  // leading rows and columns of (P|Q)
     int W;                       // beta strings per tile (64, 128, 256 or 512)
-    int A;                       // alpha singles per staged chunk (1, 2 or 4; 0 = by LDS size)
+    int A;                       // alpha singles per staged chunk (1, 2, 4 or 8; 0 = by LDS size)
     long long ntiles;
     int xg;                      // tiles per XCD run (0 / 1: blockIdx order)
+    int cm;                      // tile order in a block: 0 string-major, 1 chunk-major
     int tout;                    // 1: the descriptor is the transpose of the output's (the
                                  // determinant of lane ib_loc, string ia_loc is y[boff +
                                  // ib_loc * gn(p) + ia_loc]; the diagonal is read there too)

@@ -278,6 +278,22 @@ __device__ __forceinline__ void uniform_rows(const int2 *__restrict__ e, const d
     }
 }
 
+// position of tile `rel` of a block of nrow uniform strings x nch lane chunks: row-major
+// (the chunks of one string consecutive) or, with cm, chunk-major (consecutive strings on
+// one chunk: neighbouring strings share most excitation targets, so consecutive tiles read
+// the same vector rows over the same columns)
+__device__ __forceinline__ void tile_pos(long long rel, long long nch, long long nrow, int cm,
+                                         long long *row, long long *chunk)
+{
+    if (cm) {
+        *chunk = rel / nrow;
+        *row = rel - *chunk * nrow;
+    } else {
+        *row = rel / nch;
+        *chunk = rel - *row * nch;
+    }
+}
+
 __device__ __forceinline__ int tri32(int a, int b)
 {
     return a >= b ? ((a * (a + 1)) >> 1) + b : ((b * (b + 1)) >> 1) + a;
@@ -301,7 +317,8 @@ __global__ __launch_bounds__(W) void k_sigma_fast(FastArgs a, const XT *__restri
     const long long nq = d->s[1].gn[q];
     const long long nch = (nq + W - 1) / W;
     const long long rel = tile - a.tile_off[b];
-    const long long ia_loc = rel / nch, chunk = rel - ia_loc * nch;
+    long long ia_loc, chunk;
+    tile_pos(rel, nch, d->s[0].gn[p], a.cm, &ia_loc, &chunk);
     const long long ib_loc = chunk * W + tid;
     const bool active = ib_loc < nq;
     const long long Ia = d->s[0].gstart[p] + ia_loc;
@@ -560,7 +577,8 @@ __global__ __launch_bounds__(W) void k_sigma_fast_beta(FastArgs a, const XT *__r
     const long long nq = d->s[1].gn[q];             // alpha strings of the block
     const long long nch = (nq + W - 1) / W;
     const long long rel = tile - a.tile_off[b];
-    const long long ia_loc = rel / nch, chunk = rel - ia_loc * nch;   // beta string
+    long long ia_loc, chunk;                                          // beta string
+    tile_pos(rel, nch, d->s[0].gn[p], a.cm, &ia_loc, &chunk);
     const long long ib_loc = chunk * W + tid;                         // alpha string
     const bool active = ib_loc < nq;
     const long long Ia = d->s[0].gstart[p] + ia_loc;
@@ -686,10 +704,11 @@ template <typename AT, typename XT, int W, bool BSIDE>
 static int launch_fast(const FastArgs &a, int nvec, const XT *x, double *y, hipStream_t st)
 {
     const int A = a.A > 0 ? a.A : fast_chunk(a.m1, sizeof(AT));
+    if (A == 8) return launch_fast_a<AT, XT, W, 8, BSIDE>(a, nvec, x, y, st);
     if (A == 4) return launch_fast_a<AT, XT, W, 4, BSIDE>(a, nvec, x, y, st);
     if (A == 2) return launch_fast_a<AT, XT, W, 2, BSIDE>(a, nvec, x, y, st);
     if (A == 1) return launch_fast_a<AT, XT, W, 1, BSIDE>(a, nvec, x, y, st);
-    jci_set_error("sigma_fast: alpha chunk %d not available (1, 2 or 4)", A);
+    jci_set_error("sigma_fast: alpha chunk %d not available (1, 2, 4 or 8)", A);
     return -1;
 }
 

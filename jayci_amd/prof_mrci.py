@@ -3,8 +3,10 @@ occupied + CAS(8e,8o) + 48 external, excitation level 2; 603,651,972 determinant
 profiler run:
 
     rocprofv3 --kernel-trace --stats -d OUT -- python jayci_amd/prof_mrci.py [--fp32] [--nvec K]
+    rocprofv3 --pmc FETCH_SIZE -d OUT -- python jayci_amd/prof_mrci.py
 
-Builds the string tables, the diagonal and the per-string excitation tables on the device,
+Runs a 1 GiB device copy (the byte counters' calibration), then builds the string tables,
+the diagonal and the per-string excitation tables on the device,
 then applies the table sigma (two passes, per-block tile widths: the defaults) once to K
 seeded random vectors resident on the device (one untimed launch first) and prints the
 wall times of each stage; the kernels' own durations are read from the profiler.  Also
@@ -32,9 +34,15 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--nvirt", type=int, default=48)
     ap.add_argument("--xg", type=int, default=-1, help="tiles per XCD run (-1: default)")
+    ap.add_argument("--cm", type=int, default=-1, help="chunk-major tile order 0/1 (-1: default)")
     ap.add_argument("--split", type=int, default=-1, help="two-pass mode 0..3 (-1: default)")
+    ap.add_argument("--chunk", type=int, default=-1,
+                    help="alpha singles staged together, 1/2/4 (0: per width class)")
     a = ap.parse_args()
     ctx = J.Context(0)
+    # a 1 GiB device copy first (k_copy: 1 GiB read, 1 GiB written), the byte counters'
+    # calibration in counter runs
+    print("copy 1GiB ms %.4f" % ctx.bench_copy(1 << 30, 1), flush=True)
     t0 = time.time()
     n = 12 + a.nvirt
     m1, m2 = J.synth_integrals(n, a.seed)
@@ -45,9 +53,10 @@ def main():
     t0 = time.time()
     h = J.Hamiltonian(sp, m1, m2, J.DIRECT_TABLES)
     ctx.sync()
-    xg = h.fast_xcd(a.xg)
+    xg = h.fast_xcd(a.xg, a.cm)
     h.fast_split(a.split)
-    print("diagonal + excitation tables: %.1f s; tiles %s, split %d, XCD run %d"
+    h.fast_tiles(-1, a.chunk)
+    print("diagonal + excitation tables: %.1f s; tiles %s, split %d, XCD run, chunk-major %s"
           % (time.time() - t0, h.fast_tiles(), h.fast_split(), xg), flush=True)
     rows = np.random.default_rng(0).integers(0, N, size=64)
     cbar = float(np.mean([len(c) for c, _, _, _ in h.rows(rows)]))

@@ -36,18 +36,19 @@ def instance_spec(rng):
     return dict(na=ne, nb=ne, norbs=n, ndocc=0, nactv=n, xlvl=0, rule=1, nroots=nr)
 
 
-def mrci_spec(rng):
-    """MRCI-SD 16 electrons in 60 orbitals: 4 doubly occupied + CAS(8e,8o) + 48 external,
-    excitation level 2, MRCI-SD rule (603,651,972 determinants), roots ~ U{1..3}."""
-    return dict(na=8, nb=8, norbs=60, ndocc=4, nactv=8, xlvl=2, rule=1,
+def mrci_spec(rng, nvirt=48):
+    """MRCI-SD 16 electrons in 12 + nvirt orbitals: 4 doubly occupied + CAS(8e,8o) + nvirt
+    external, excitation level 2, MRCI-SD rule (603,651,972 determinants at 48 external),
+    roots ~ U{1..3}."""
+    return dict(na=8, nb=8, norbs=12 + nvirt, ndocc=4, nactv=8, xlvl=2, rule=1,
                 nroots=int(rng.integers(1, 4)))
 
 
-def make_stream(nreq, rate, seed, spec_fn=instance_spec, mrci_share=0.0):
+def make_stream(nreq, rate, seed, spec_fn=instance_spec, mrci_share=0.0, mrci_nvirt=48):
     """Seeded Poisson stream: list of (arrival_s, spec, integral_seed).  With mrci_share > 0
-    every request is an MRCI-SD 16e/60o solve with that probability (drawn from its own
-    generator, so the full-CI requests are those of the stream without the share), and the
-    number of such requests is round(mrci_share * nreq), at least one."""
+    round(mrci_share * nreq) requests, at least one, chosen by their own generator (so the
+    full-CI requests are those of the stream without the share), are MRCI-SD solves of 16
+    electrons in 12 + mrci_nvirt orbitals (mrci_spec)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -58,7 +59,7 @@ def make_stream(nreq, rate, seed, spec_fn=instance_spec, mrci_share=0.0):
         mr = np.random.default_rng(seed + 7)
         nm = max(1, int(round(mrci_share * nreq)))
         for i in sorted(mr.choice(nreq, size=nm, replace=False)):
-            out[i] = (out[i][0], mrci_spec(mr), out[i][2])
+            out[i] = (out[i][0], mrci_spec(mr, mrci_nvirt), out[i][2])
     return out
 
 
@@ -77,17 +78,17 @@ def lane_vector_bytes(N, krymax, kb):
     return N * (8 * (2 * krymax + 2 * kb) + 12 * kb + 8 + 120)
 
 
-def lane_solve_params(spec, N, restol=1e-6, etol=1e-9, buftol=1e-3, budget=None):
+def lane_solve_params(spec, N, restol=1e-6, etol=1e-9, buftol=1e-3, budget=None, kfac=4):
     """Davidson settings of a lane request (a space far larger than the batched ones): one
-    buffer pair for a single root, none above; a subspace of four times the tracked pairs (at
-    least 8), or the largest that keeps lane_vector_bytes within `budget`, but not below
+    buffer pair for a single root, none above; a subspace of kfac times the tracked pairs
+    (at least 8), or the largest that keeps lane_vector_bytes within `budget`, but not below
     twice the tracked pairs (at least 6); GD+k restarts, diagonal guess.  With three tracked
     pairs the subspace of 12 converges in 71 iterations where 9 takes 99 and 6 stalls
     (MRCI-SD 16e/20o, 3 roots), so memory goes to the subspace first."""
     nr = spec["nroots"]
     nbuf = 1 if nr == 1 else 0
     nt = nr + nbuf
-    kmax, kmin = max(4 * nt, 8), max(2 * nt, 6)
+    kmax, kmin = max(kfac * nt, 8), max(2 * nt, 6)
     while budget is not None and kmax > kmin and lane_vector_bytes(N, kmax, nt) > budget:
         kmax -= 1
     return dict(nroots=nr, restol=restol, krymin=min(nt + 1, N), krymax=min(kmax, N),
@@ -133,7 +134,8 @@ def percentile(xs, q):
 
 class Server:
     def __init__(self, ctx, capacity=64, budget_bytes=200e9, max_norbs=16, precision=PREC_FP64,
-                 restol=1e-6, mixed_switch=1e-4, graphs=True, nbuffer=1, pool_bytes=24 << 30):
+                 restol=1e-6, mixed_switch=1e-4, graphs=True, nbuffer=1, pool_bytes=24 << 30,
+                 lane_cu_percent=100):
         self.ctx = ctx
         self.capacity = capacity
         self.max_norbs = max_norbs
@@ -148,8 +150,10 @@ class Server:
         # thread with a context of its own, beside the steps of the running batch
         self.bctx = Context(ctx.device)
         self.builder = ThreadPoolExecutor(max_workers=1)
-        # the lane of requests the batch does not take (see the module docstring)
-        self.lctx = Context(ctx.device)
+        # the lane of requests the batch does not take (see the module docstring); with
+        # lane_cu_percent < 100 its kernels keep to that share of the compute units, so that
+        # the batch's short steps do not queue behind a lane sigma's work-groups
+        self.lctx = Context(ctx.device, lane_cu_percent)
         self.lane = ThreadPoolExecutor(max_workers=1)
         self.t0 = None
 
