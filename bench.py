@@ -303,8 +303,7 @@ def bench_stream(J, ctx, args, rate, seed):
         # the opposite-spin pool shrinks to 8 GiB when MRCI-SD requests share the budget
         srv = serve.Server(ctx, capacity=256, budget_bytes=200e9, max_norbs=LANE_NORBS,
                            precision=precision, restol=1e-6,
-                           pool_bytes=(8 << 30) if nmrci else (24 << 30),
-                           lane_cu_percent=args.lane_cu)
+                           pool_bytes=(8 << 30) if nmrci else (24 << 30))
         try:
             tiles[precision] = srv.tune_tiles(stream[big][1], stream[big][2])
             ctx.sync()
@@ -603,8 +602,6 @@ def main():
                          "allows (twice at the least)")
     ap.add_argument("--mrci-sigma-bench", action="store_true",
                     help="mrci: also time single sigma launches of 1 and the block's vectors")
-    ap.add_argument("--lane-cu", type=int, default=100,
-                    help="served: percent of the compute units the MRCI lane's kernels use")
     ap.add_argument("--mrci-share", type=float, default=0.1,
                     help="served: share of MRCI-SD 16e/(12 + mrci-nvirt)o requests in the "
                          "stream")

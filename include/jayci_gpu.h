@@ -190,10 +190,6 @@ int jci_device_count(int *n);
 int jci_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 
 int jci_ctx_create(int device, jci_ctx **out);
-/* a context whose stream runs its kernels on cu_percent % of the device's compute units
- * (1 .. 100; spread evenly over the CU numbering), the rest left to other streams: a long
- * solve beside a stream of short ones keeps the latter from waiting behind its work-groups */
-int jci_ctx_create_cus(int device, int cu_percent, jci_ctx **out);
 void jci_ctx_destroy(jci_ctx *ctx);
 int jci_ctx_sync(jci_ctx *ctx);
 /* the HIP stream the context's work is queued on (a hipStream_t) */

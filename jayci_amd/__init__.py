@@ -58,7 +58,6 @@ _sig("jci_last_error", ctypes.c_char_p, [])
 _sig("jci_device_count", _i, [ctypes.POINTER(_i)])
 _sig("jci_device_memory", _i, [_i, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)])
 _sig("jci_ctx_create", _i, [_i, ctypes.POINTER(_P)])
-_sig("jci_ctx_create_cus", _i, [_i, _i, ctypes.POINTER(_P)])
 _sig("jci_ctx_destroy", None, [_P])
 _sig("jci_ctx_sync", _i, [_P])
 _sig("jci_ctx_stream", _P, [_P])
@@ -195,15 +194,9 @@ def device_count():
 
 
 class Context:
-    def __init__(self, device=0, cu_percent=100):
-        """A device context and its stream; cu_percent < 100 runs the stream's kernels on that
-        share of the compute units (jci_ctx_create_cus)."""
+    def __init__(self, device=0):
         h = _P()
-        if cu_percent < 100:
-            _check(_L.jci_ctx_create_cus(device, cu_percent, ctypes.byref(h)),
-                   "jci_ctx_create_cus")
-        else:
-            _check(_L.jci_ctx_create(device, ctypes.byref(h)), "jci_ctx_create")
+        _check(_L.jci_ctx_create(device, ctypes.byref(h)), "jci_ctx_create")
         self._h = h
         self.device = device
 

@@ -78,31 +78,16 @@ JCI_API int jci_device_memory(int device, size_t *free_b, size_t *total_b)
     return 0;
 }
 
-JCI_API int jci_ctx_create_cus(int device, int cu_percent, jci_ctx **out)
+JCI_API int jci_ctx_create(int device, jci_ctx **out)
 {
     *out = nullptr;
     int n = 0;
     jci_device_count(&n);
     if (n <= 0) { jci_set_error("no HIP device visible"); return -2; }
     if (device < 0 || device >= n) { jci_set_error("device %d out of range", device); return -1; }
-    if (cu_percent < 1 || cu_percent > 100) {
-        jci_set_error("CU share %d%% not in 1 .. 100", cu_percent);
-        return -1;
-    }
     JCI_CHECK(hipSetDevice(device));
     hipStream_t st = nullptr;
-    if (cu_percent < 100) {
-        // every CU i with floor((i + 1) p / 100) > floor(i p / 100): p % of them, spread
-        // evenly over the device's CU numbering (and so over its XCDs)
-        int ncu = 0;
-        JCI_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
-        std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
-        for (int i = 0; i < ncu; i++)
-            if ((i + 1) * cu_percent / 100 > i * cu_percent / 100) mask[i / 32] |= 1u << (i % 32);
-        JCI_CHECK(hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data()));
-    } else {
-        JCI_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    }
+    JCI_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     jci_ctx *c = (jci_ctx *)calloc(1, sizeof(jci_ctx));
     c->device = device;
     c->stream = st;
@@ -118,11 +103,6 @@ JCI_API int jci_ctx_create_cus(int device, int cu_percent, jci_ctx **out)
     if (rc) return rc;
     *out = c;
     return 0;
-}
-
-JCI_API int jci_ctx_create(int device, jci_ctx **out)
-{
-    return jci_ctx_create_cus(device, 100, out);
 }
 
 static void ctx_free(jci_ctx *c)

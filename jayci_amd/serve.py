@@ -134,8 +134,7 @@ def percentile(xs, q):
 
 class Server:
     def __init__(self, ctx, capacity=64, budget_bytes=200e9, max_norbs=16, precision=PREC_FP64,
-                 restol=1e-6, mixed_switch=1e-4, graphs=True, nbuffer=1, pool_bytes=24 << 30,
-                 lane_cu_percent=100):
+                 restol=1e-6, mixed_switch=1e-4, graphs=True, nbuffer=1, pool_bytes=24 << 30):
         self.ctx = ctx
         self.capacity = capacity
         self.max_norbs = max_norbs
@@ -150,10 +149,8 @@ class Server:
         # thread with a context of its own, beside the steps of the running batch
         self.bctx = Context(ctx.device)
         self.builder = ThreadPoolExecutor(max_workers=1)
-        # the lane of requests the batch does not take (see the module docstring); with
-        # lane_cu_percent < 100 its kernels keep to that share of the compute units, so that
-        # the batch's short steps do not queue behind a lane sigma's work-groups
-        self.lctx = Context(ctx.device, lane_cu_percent)
+        # the lane of requests the batch does not take (see the module docstring)
+        self.lctx = Context(ctx.device)
         self.lane = ThreadPoolExecutor(max_workers=1)
         self.t0 = None
 
@@ -251,20 +248,27 @@ class Server:
             # admission, first come first served within slots and HBM budget: a batch request
             # is built on the worker, then added to the batch here once its build has
             # finished (waiting for it only when nothing else could step); a lane request
-            # starts on the lane when the lane is free
-            while waiting and used + waiting[0].bytes <= self.budget:
-                r = waiting[0]
-                if self._on_lane(r.spec):
-                    if lane is not None:
-                        break
-                    waiting.pop(0)
+            # starts on the lane when the lane is free.  A lane request waiting for the busy
+            # lane does not hold up the batch requests behind it (each kind keeps its order);
+            # the first request that does not fit the budget stops admission.
+            i = 0
+            while i < len(waiting):
+                r = waiting[i]
+                on_lane = self._on_lane(r.spec)
+                if on_lane and lane is not None:
+                    i += 1
+                    continue
+                if used + r.bytes > self.budget:
+                    break
+                if on_lane:
+                    waiting.pop(i)
                     used += r.bytes
                     r.future = self.lane.submit(self._solve_lane, r)
                     lane = r
                     continue
                 if len(active) + len(building) >= self.capacity:
                     break
-                waiting.pop(0)
+                waiting.pop(i)
                 used += r.bytes
                 r.future = self.builder.submit(self._build, r)
                 building.append(r)
