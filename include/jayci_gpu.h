@@ -258,8 +258,8 @@ typedef struct {
     int precision;        /* JCI_PREC_FP64 or JCI_PREC_MIXED */
     double mixed_switch;  /* residual norm below which FP32 sigma hands over to FP64 (it
                            * also hands over when the largest residual, below 100 times
-                           * this, has made no new 10 % minimum for 6 iterations: the FP32
-                           * noise floor) */
+                           * this or below 1e-2, has made no new 10 % minimum for 6
+                           * iterations: the FP32 noise floor) */
     int sigma_fp64, sigma_fp32; /* JCI_SIGMA_* used in each phase (0 = automatic) */
     int guess;            /* JCI_GUESS_* */
     double totfrze;       /* added to the eigenvalues (nuclear repulsion + frozen core) */

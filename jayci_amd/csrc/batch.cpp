@@ -982,7 +982,7 @@ JCI_API int jci_batch_step(jci_batch *b, int *nactive)
         if (s.phase32 && rmax < JCI_MIXED_GAIN * s.best32) { s.best32 = rmax; s.it_best32 = s.iters; }
         if (s.phase32 && (rmax < s.p.mixed_switch || s.iters >= s.p.maxiter ||
                           (s.iters - s.it_best32 >= JCI_MIXED_STALL &&
-                           s.best32 < JCI_MIXED_NEAR * s.p.mixed_switch))) {
+                           s.best32 < jci_mixed_near(s.p.mixed_switch)))) {
             s.phase32 = 0;
             handover.push_back(act[t]);
             continue;

@@ -1898,7 +1898,7 @@ int jci_solve_core(SolveOp *op, const jci_solve_params *p, const double *guess, 
         if (phase32 && rmax < JCI_MIXED_GAIN * best32) { best32 = rmax; it_best32 = it; }
         if (phase32 && (rmax < p->mixed_switch || it == p->maxiter ||
                         (it - it_best32 >= JCI_MIXED_STALL &&
-                         best32 < JCI_MIXED_NEAR * p->mixed_switch))) {
+                         best32 < jci_mixed_near(p->mixed_switch)))) {
             // hand over to FP64 (below the switch residual, or stalled at the FP32 sigma's
             // noise floor above it): keep the lowest kc Ritz vectors of the FP32 phase (the
             // basis itself is FP64) and recompute their sigma in FP64

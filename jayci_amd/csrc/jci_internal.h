@@ -9,13 +9,13 @@
 #define JCI_MAXBASIS 64      // Davidson subspace limit (krymax)
 #define JCI_MAXK 8           // vectors per sigma launch
 // mixed precision: the FP32 phase also hands over to FP64 when its largest residual, already
-// below JCI_MIXED_NEAR times the switch residual, has not reached JCI_MIXED_GAIN times its
-// best for JCI_MIXED_STALL iterations (the FP32 sigma's noise floor lies above the switch
-// residual: energies of a few hundred Ha put it near 1e-4; far above the floor a slow
-// stretch of the Davidson residual is not a stall)
+// below jci_mixed_near(switch residual), has not reached JCI_MIXED_GAIN times its best for
+// JCI_MIXED_STALL iterations (the FP32 sigma's noise floor lies above the switch residual:
+// energies of a few hundred Ha put it near 1e-4; far above the floor a slow stretch of the
+// Davidson residual is not a stall)
 #define JCI_MIXED_GAIN 0.9
 #define JCI_MIXED_STALL 6
-#define JCI_MIXED_NEAR 100.0
+static inline double jci_mixed_near(double sw) { return sw * 100.0 > 1e-2 ? sw * 100.0 : 1e-2; }
 
 #define JCI_CHECK(x)                                                                   \
     do {                                                                               \
