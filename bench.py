@@ -231,6 +231,43 @@ def roofline_dense(J, ctx, spec, seed):
     return out
 
 
+def roofline_table(J, ctx, spec, seed):
+    """Matrix-free table sigma (k_block_transpose, k_sigma_fast and k_sigma_fast_beta
+    launches of one pass, default settings), FP64 and FP32, one vector, on one MRCI-SD
+    instance of the stream.  HBM-bound by its governing definition: the compulsory bytes of
+    a pass that cannot materialise its pair list, N*(2*k*b + 8) (vector read, sigma
+    written, diagonal read; b bytes per vector element), over the HIP-event time of one
+    pass; the connected entries per second (N times the mean connections of 64 sampled
+    rows) beside it.  `traffic` is the rocprof byte count of the same pass."""
+    import numpy as np
+    sp, h = make_instance(J, ctx, spec, seed, J.DIRECT_TABLES)
+    N = sp.ndets
+    rows = np.random.default_rng(0).integers(0, N, size=64)
+    cbar = float(np.mean([len(c) for c, _, _, _ in h.rows(rows)]))
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "sigma_tables_mrcisd_16e20o_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("ndets") == N:
+            traffic = {64: t.get("bytes_per_launch_fp64")}
+    out = {}
+    for prec, meth, b in ((64, J.SIGMA_FAST64, 8), (32, J.SIGMA_FAST32, 4)):
+        ms = h.bench_sigma(meth, 1, 5)
+        byt = N * (2 * b + 8)
+        ach = byt / (ms * 1e-3) / 1e9
+        out[prec] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s",
+                     "frac": ach / PEAK_HBM, "traffic": traffic.get(prec),
+                     "kernel": "table sigma %s (k_block_transpose + k_sigma_fast + "
+                               "k_sigma_fast_beta), N=%d (MRCI-SD 16e/%do)"
+                               % ("FP64" if prec == 64 else "FP32", N, spec["norbs"]),
+                     "ms_per_launch": ms, "algorithmic_bytes": byt,
+                     "byte_count": "N*(2*k*b + 8): vector read, sigma written, diagonal read",
+                     "connected_entries_per_s": N * cbar / (ms * 1e-3)}
+    h.close()
+    sp.close()
+    return out
+
+
 def run_plain(J, ctx, stream, restol, precision):
     """One request at a time in arrival order (no batching).  Returns (results, wall)."""
     out = []
@@ -394,6 +431,7 @@ def bench_stream(J, ctx, args, rate, seed):
         "mixed_over_plain": wall_p / wall_m,
         "_cpu": (len(fci_plain), elements),
         "_big": (stream[big][1], stream[big][2]),
+        "_mrci": next(((sp_, sd_) for _, sp_, sd_ in stream if sp_["norbs"] > LANE_NORBS), None),
     }
 
 
@@ -646,9 +684,19 @@ def main():
         log("same sigma by the dense-string path: %.3f ms against %.3f ms for the pair bytes at "
             "HBM peak" % (plain_roof["dense_string_fp64_ms"], plain_roof["roofline_ms_at_peak"]))
     dense = roofline_dense(J, ctx, bspec, bseed)
-    roof = dense[64]
+    roof, roof32 = dense[64], dense[32]
     log("dense-string sigma FP64 %.1f TFLOP/s (%.3f), FP32 %.1f TFLOP/s (%.3f)"
         % (dense[64]["achieved"], dense[64]["frac"], dense[32]["achieved"], dense[32]["frac"]))
+    mrci = res.pop("_mrci", None)
+    table = None
+    if mrci is not None:
+        # with the MRCI-SD share the stream's time is the lane's table sigma: it is the
+        # dominant kernel, the dense-string sigma of the full-CI requests is reported beside it
+        table = roofline_table(J, ctx, mrci[0], mrci[1])
+        roof, roof32 = table[64], table[32]
+        log("table sigma FP64 %.3f ms per pass, %.2f GB/s compulsory (%.5f of HBM peak), "
+            "%.3g entries/s" % (table[64]["ms_per_launch"], table[64]["achieved"],
+                                table[64]["frac"], table[64]["connected_entries_per_s"]))
     cpu = None
     if not args.no_cpu:
         try:
@@ -667,8 +715,10 @@ def main():
     if res.get("dropin"):
         line["dropin"] = res["dropin"]
     line.update({"mixed_over_plain": res["mixed_over_plain"], "roofline": roof,
-                 "roofline_fp32": dense[32], "roofline_pair_list": plain_roof,
+                 "roofline_fp32": roof32, "roofline_pair_list": plain_roof,
                  "cpu_baseline": cpu})
+    if table is not None:
+        line.update({"roofline_dense": dense[64], "roofline_dense_fp32": dense[32]})
     print(json.dumps(line))
 
 
