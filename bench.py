@@ -255,8 +255,8 @@ def roofline_table(J, ctx, spec, seed):
         ms = h.bench_sigma(meth, 1, 5)
         byt = N * (2 * b + 8)
         ach = byt / (ms * 1e-3) / 1e9
-        out[prec] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s",
-                     "frac": ach / PEAK_HBM, "traffic": traffic.get(prec),
+        out[prec] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": ach * 1e9 / PEAK_HBM, "traffic": traffic.get(prec),
                      "kernel": "table sigma %s (k_block_transpose + k_sigma_fast + "
                                "k_sigma_fast_beta), N=%d (MRCI-SD 16e/%do)"
                                % ("FP64" if prec == 64 else "FP32", N, spec["norbs"]),
@@ -385,6 +385,28 @@ def bench_stream(J, ctx, args, rate, seed):
     err_p = max(err_of(i, ev) for i, _, ev, _ in plain)
     worst_m = worst([(r.idx, err_of(r.idx, r.energies), r.stats["rnorm"]) for r in done_m])
     worst_p = worst([(i, err_of(i, ev), st["rnorm"]) for i, _, ev, st in plain])
+    # the oracle's eigenvalues of the stream's small full-CI requests (tests/golden, made by
+    # its script there with the oracle's CPU Davidson at residual 1e-9), where the stream is
+    # the one the fixture was made for and the request's spec matches
+    gold = {}
+    gpath = os.path.join(ROOT, "tests", "golden", "stream%d_energies.json" % seed)
+    if rate and len(stream) == 64 and os.path.exists(gpath):
+        for g in json.load(open(gpath))["requests"]:
+            i = g["request"]
+            if stream[i][1] == g["spec"] and stream[i][2] == g["seed"]:
+                gold[i] = g["energies"]
+
+    def err_gold(items):
+        errs = [max(abs(ev[k] - gold[i][k]) for k in range(len(ev))) for i, ev in items
+                if i in gold]
+        return {"requests": len(errs), "max_abs_err_Ha": max(errs) if errs else None}
+    gold_m = err_gold([(r.idx, r.energies) for r in done_m])
+    gold_b = err_gold([(r.idx, r.energies) for r in done_b])
+    gold_p = err_gold([(i, ev) for i, _, ev, _ in plain])
+    if gold:
+        log("against the oracle's energies (%d requests): mixed %.2e, served fp64 %.2e, plain "
+            "%.2e Ha" % (len(gold), gold_m["max_abs_err_Ha"], gold_b["max_abs_err_Ha"],
+                         gold_p["max_abs_err_Ha"]))
     lat_m = [r.t_done - r.arrival for r in done_m]
     lat_b = [r.t_done - r.arrival for r in done_b]
     lat_p = [lat for _, lat, _, _ in plain]
@@ -416,18 +438,23 @@ def bench_stream(J, ctx, args, rate, seed):
                   "converged": sum(1 for r in done_m if r.stats["done"] == 1),
                   "sigma_s": sig_m, "step_phases_s": phases.get(J.PREC_MIXED),
                   "tuned_tiles_fp64_fp32": tiles.get(J.PREC_MIXED),
-                  "worst": worst_m,
+                  "worst": worst_m, "vs_oracle": gold_m,
                   "path": "served: continuous batching, mixed precision"},
         "plain_fp64": {"solves_per_s": len(plain) / wall_p, "p50_s": percentile(lat_p, 0.5),
                        "p99_s": percentile(lat_p, 0.99), "max_abs_err_Ha": err_p,
-                       "worst": worst_p,
+                       "worst": worst_p, "vs_oracle": gold_p,
                        "path": "one request at a time in arrival order, FP64"},
         "served_fp64": {"solves_per_s": len(done_b) / wall_b, "p50_s": percentile(lat_b, 0.5),
                         "p99_s": percentile(lat_b, 0.99), "max_abs_err_Ha": err_b,
                         "sigma_s": sig_b, "step_phases_s": phases.get(J.PREC_FP64),
+                        "vs_oracle": gold_b,
                         "path": "served: continuous batching, FP64"},
         "reference_solves": {"restol": 1e-8, "largest_final_residual": ref_r,
-                             "what": "FP64 GPU solve of every request, 2 buffer roots"},
+                             "what": "FP64 GPU solve of every request, 2 buffer roots "
+                                     "(max_abs_err_Ha); vs_oracle: the oracle's CPU Davidson "
+                                     "energies of the requests of at most 30,000 "
+                                     "determinants (tests/golden/stream%d_energies.json)"
+                                     % seed},
         "mixed_over_plain": wall_p / wall_m,
         "_cpu": (len(fci_plain), elements),
         "_big": (stream[big][1], stream[big][2]),
