@@ -245,33 +245,22 @@ class Server:
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
                 waiting.append(reqs[nxt])
                 nxt += 1
-            # admission, first come first served within slots and HBM budget: a batch request
-            # is built on the worker, then added to the batch here once its build has
-            # finished (waiting for it only when nothing else could step); a lane request
-            # starts on the lane when the lane is free.  A lane request waiting for the busy
-            # lane does not hold up the batch requests behind it (each kind keeps its order);
-            # the first request that does not fit the budget stops admission.
-            i = 0
-            while i < len(waiting):
+            # admission (admit): a batch request is built on the worker, then added to the
+            # batch here once its build has finished (waiting for it only when nothing else
+            # could step); a lane request starts on the lane when the lane is free
+            lane_pick, picks, used = admit(waiting, lambda q: self._on_lane(q.spec),
+                                           lane is not None, used, self.budget,
+                                           self.capacity - len(active) - len(building))
+            if lane_pick is not None:
+                lane = waiting[lane_pick]
+                lane.future = self.lane.submit(self._solve_lane, lane)
+            for i in picks:
                 r = waiting[i]
-                on_lane = self._on_lane(r.spec)
-                if on_lane and lane is not None:
-                    i += 1
-                    continue
-                if used + r.bytes > self.budget:
-                    break
-                if on_lane:
-                    waiting.pop(i)
-                    used += r.bytes
-                    r.future = self.lane.submit(self._solve_lane, r)
-                    lane = r
-                    continue
-                if len(active) + len(building) >= self.capacity:
-                    break
-                waiting.pop(i)
-                used += r.bytes
                 r.future = self.builder.submit(self._build, r)
                 building.append(r)
+            if lane_pick is not None or picks:
+                taken = set(picks) | ({lane_pick} if lane_pick is not None else set())
+                waiting = [r for j, r in enumerate(waiting) if j not in taken]
             if lane is not None and lane.future.done():
                 lane.future.result()
                 used -= lane.bytes
@@ -310,6 +299,32 @@ class Server:
                     used -= r.bytes
                     done.append(r)
         return done
+
+
+def admit(waiting, on_lane, lane_busy, used, budget, slots_free):
+    """One admission pass over the waiting queue (objects with .bytes), first come first
+    served within the HBM budget and the batch's free slots: returns (index of the request
+    that starts on the lane or None, indices of the requests for the batch in queue order,
+    bytes in use after them).  A lane request waiting for the busy lane does not hold up
+    the batch requests behind it (each kind keeps its order); the first request that does
+    not fit the budget stops the pass, and so does a batch request without a free slot."""
+    lane_pick, picks = None, []
+    for i, r in enumerate(waiting):
+        lane_req = on_lane(r)
+        if lane_req and (lane_busy or lane_pick is not None):
+            continue
+        if used + r.bytes > budget:
+            break
+        if lane_req:
+            lane_pick = i
+            used += r.bytes
+            continue
+        if slots_free <= 0:
+            break
+        picks.append(i)
+        used += r.bytes
+        slots_free -= 1
+    return lane_pick, picks, used
 
 
 def summarize(done, wall):

@@ -69,3 +69,25 @@ def test_stream_mrci_share_and_lane_settings():
                 or kb["krymax"] == max(2 * nt, 6))
         assert serve.lane_bytes(spec, N, budget=bud) == serve.lane_vector_bytes(
             N, kb["krymax"], nt)
+
+
+def test_admission_order_budget_and_lane():
+    """serve.admit: a lane request waiting for the busy lane does not hold up the batch
+    requests behind it; the budget and the batch's slots stop the pass in queue order."""
+    from types import SimpleNamespace as R
+    from jayci_amd import serve
+    lane = lambda r: r.lane
+    q = [R(lane=True, bytes=50), R(lane=False, bytes=10), R(lane=False, bytes=10)]
+    # lane busy: the two batch requests behind the lane request go ahead
+    assert serve.admit(q, lane, True, 0, 100, 8) == (None, [1, 2], 20)
+    # lane free: the lane request starts, both batch requests too
+    assert serve.admit(q, lane, False, 0, 100, 8) == (0, [1, 2], 70)
+    # budget: the first request that does not fit stops the pass, even if later ones fit
+    q2 = [R(lane=False, bytes=90), R(lane=False, bytes=5)]
+    assert serve.admit(q2, lane, False, 20, 100, 8) == (None, [], 20)
+    # slots: one free slot takes the first batch request only
+    assert serve.admit(q2, lane, False, 0, 100, 1) == (None, [0], 90)
+    # only one lane request starts per pass; a second waits behind it
+    q3 = [R(lane=False, bytes=1), R(lane=True, bytes=1), R(lane=True, bytes=1),
+          R(lane=False, bytes=1)]
+    assert serve.admit(q3, lane, False, 0, 100, 8) == (1, [0, 3], 3)
