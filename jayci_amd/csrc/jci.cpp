@@ -1041,10 +1041,11 @@ static int fast_run(jci_ham *h, int method, int nvec, const double *x, double *y
     // one launch of the chosen width, or one per width class (fast_w == 3); two passes: per
     // class and orientation (set_orient)
     // alpha chunk per class unless one is set: FP64 1 for rows of up to 128 strings, 2 up
-    // to 256, 4 above; FP32 4 throughout (LDS per work-group grows with the chunk and the
+    // to 256, 4 above; FP32 2, 2, 4, 4 (LDS per work-group grows with the chunk and the
     // element size; measured at 16e/60o: FP64 7.44 s against 8.75 s with 2 and 8.87 s with
-    // 8 throughout, FP32 5.86 s against 6.02 s with 2/2/4/8)
-    const int cls_a[4] = {prec == 64 ? 1 : 4, prec == 64 ? 1 : 4, prec == 64 ? 2 : 4, 4};
+    // 8 throughout; FP32 5.86 s with 4 throughout against 6.02 s with 2/2/4/8, while at
+    // 16e/20o 4 throughout took 42.2 ms against 38.3 ms)
+    const int cls_a[4] = {prec == 64 ? 1 : 2, prec == 64 ? 1 : 2, prec == 64 ? 2 : 4, 4};
     const int A_set = h->fast[0].A;
     FastArgs a[8], t[8];
     int n = 1;
