@@ -280,8 +280,8 @@ struct jci_ham {
     int fast_w;                 // index into fast[] used by the FAST methods
     // tile-width classes: each block in one launch of the width that fits its strings per
     // row (64 .. 512), used when fast_w == 3
-    FastArgs fcls[4], fclsT[4];
-    long long *d_cls_off[4], *d_clsT_off[4];
+    FastArgs fcls[4];
+    long long *d_cls_off[4];
     // the beta side as a second pass over the block-transposed vector (jk_sigma_fast_split)
     int fast_split;             // 0: one fused kernel; two passes: 1 every block in its own
                                 // orientation, 2 blocks of short beta rows transposed,
@@ -651,28 +651,21 @@ static int build_fast(jci_ham *h, int W)
         // (512 above 256)
         const int cw[4] = {64, 128, 256, 512};
         auto cls_of = [&](long long nq) { return nq <= 64 ? 0 : nq <= 128 ? 1 : nq <= 256 ? 2 : 3; };
+        // (the fused kernel's launches; the two-pass launches come from set_orient)
         for (int ci = 0; ci < 4; ci++) {
-            std::vector<long long> toff(D.nblk + 1, 0), toffT(D.nblk + 1, 0);
+            std::vector<long long> toff(D.nblk + 1, 0);
             for (int b = 0; b < D.nblk; b++) {
                 long long na = D.s[0].gn[D.bp[b]], nq = D.s[1].gn[D.bq[b]];
                 toff[b + 1] = toff[b] + (cls_of(nq) == ci ? na * ((nq + cw[ci] - 1) / cw[ci]) : 0);
-                toffT[b + 1] = toffT[b] + (cls_of(na) == ci ? nq * ((na + cw[ci] - 1) / cw[ci]) : 0);
             }
             JCI_CHECK(hipMalloc(&h->d_cls_off[ci], sizeof(long long) * (D.nblk + 1)));
-            JCI_CHECK(hipMalloc(&h->d_clsT_off[ci], sizeof(long long) * (D.nblk + 1)));
             JCI_CHECK(hipMemcpyAsync(h->d_cls_off[ci], toff.data(), sizeof(long long) * (D.nblk + 1),
-                                     hipMemcpyHostToDevice, c->stream));
-            JCI_CHECK(hipMemcpyAsync(h->d_clsT_off[ci], toffT.data(), sizeof(long long) * (D.nblk + 1),
                                      hipMemcpyHostToDevice, c->stream));
             JCI_CHECK(hipStreamSynchronize(c->stream));
             h->fcls[ci] = h->fast[2];
             h->fcls[ci].W = cw[ci];
             h->fcls[ci].tile_off = h->d_cls_off[ci];
             h->fcls[ci].ntiles = toff[D.nblk];
-            h->fclsT[ci] = h->fastT[2];
-            h->fclsT[ci].W = cw[ci];
-            h->fclsT[ci].tile_off = h->d_clsT_off[ci];
-            h->fclsT[ci].ntiles = toffT[D.nblk];
         }
         if (multi) h->fast_w = 3;
         for (int i = 0; i < 16; i++)
@@ -746,7 +739,7 @@ JCI_API void jci_ham_destroy(jci_ham *h)
     hipFree(h->d_m2sq);
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tile_off[wi]);
     for (int wi = 0; wi < 3; wi++) hipFree(h->d_tileT_off[wi]);
-    for (int ci = 0; ci < 4; ci++) { hipFree(h->d_cls_off[ci]); hipFree(h->d_clsT_off[ci]); }
+    for (int ci = 0; ci < 4; ci++) hipFree(h->d_cls_off[ci]);
     for (int i = 0; i < 16; i++) hipFree(h->d_ooff[i]);
     hipFree(h->d_descT);
     hipFree(h->d_tt_off);
